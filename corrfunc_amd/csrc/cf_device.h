@@ -1,0 +1,72 @@
+// cf_device.h -- shared host/device declarations of the MI355X pair-counting engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cfamd {
+
+enum Mode { MODE_DD = 0, MODE_RPPI = 1, MODE_SMU = 2, MODE_XI = 3, MODE_WP = 4 };
+
+// FP64 rounding orders of the reference per-ISA kernels.
+enum R2Form {
+    R2_SUM_XY_Z = 0,      // (dx*dx + dy*dy) + dz*dz          fallback / SSE / AVX (DD, DDsmu)
+    R2_FMA_DZ_DY_DX = 1,  // fma(dz,dz,fma(dy,dy,dx*dx))      AVX512F DD, xi
+    R2_X_SUM_YZ = 2,      // dx*dx + (dy*dy + dz*dz)          AVX/AVX2 xi
+    R2_FMA_DX_DY_DZ = 3,  // fma(dx,dx,fma(dy,dy,dz*dz))      AVX512F DDsmu
+    RP_SUM = 10,          // dx*dx + dy*dy                    fallback rp
+    RP_FMA = 11           // fma(dy,dy,dx*dx)                 AVX512F rp_pi / wp, AVX2 wp
+};
+
+// Number of z-slabs each cell is bucketed into by the gridding sort.
+constexpr int kSlabs = 16;
+
+// One neighbour-cell pair of the work list (64 bytes). Mirrors the fields of
+// the reference's struct cell_pair (utils/cell_pair.h.src:20-37); the wrap
+// offsets are stored as signs so off = sign * wrap is exact.
+struct CellPairRec {
+    int c1, c2;
+    signed char wx, wy, wz, same;
+    int pad;
+    double mdx, mdy, mdz;   // min_dx, min_dy, min_dz
+    double cx, cy, cz;      // closest_x1, closest_y1, closest_z1
+};
+static_assert(sizeof(CellPairRec) == 64, "CellPairRec must be 64 bytes");
+
+// A gridded particle set resident in HBM, sorted by (cell, z-slab).
+struct DevLattice {
+    int nx, ny, nz;
+    int64_t ncells;
+    int64_t n;
+    double xmin, ymin, zmin;
+    double xbs, ybs, zbs;      // cell sizes
+    double xinv, yinv, zinv;   // 1/cell size (0 if the size is 0)
+    double *x, *y, *z, *w;     // FP64 positions / weight, cell order
+    float4 *f4;                // cell-local FP32 (x - ox, y - oy, z - oz, 0), ox = xmin + ix*xbs
+    int *bstart;               // [ncells*kSlabs + 1] bucket starts
+    double *bounds;            // [ncells*6] x lo,hi, y lo,hi, z lo,hi
+};
+
+// Parameters of one pair-counting launch.
+struct PairParams {
+    int mode;          // Mode
+    int r2form;        // R2Form
+    int float_index;   // AVX512F flattened (rp,pi)/(s,mu) index rule
+    int nbin;          // edges
+    int n2;            // npibin / nmu_bins (0 for 1-D modes)
+    int nslots;        // histogram slots
+    int need_avg, need_w;
+    int autocorr;
+    int sorted_edges;  // rupp_sqr non-decreasing -> binary search
+    double sqr_rmax, sqr_rmin;
+    double pimax;      // DD/xi: rmax; rp_pi/wp: pimax; s_mu: smax*mu_max
+    double inv_d2, sqr_mu_max;
+    double max_all_dz_same;  // same-cell z window (DD/xi/s_mu)
+    double xwrap, ywrap, zwrap;
+    double near_r2, near_dz; // edge windows below which pruning replay is skipped
+    // FP32 filter thresholds
+    float hi2_f, lo2_f, zreach_f, pad_f;
+    double zreach;           // z reach used for the slab window
+    double slab_margin;
+};
+
+}  // namespace cfamd

@@ -1,0 +1,1133 @@
+// cf_engine.hip -- host side of the MI355X theory pair-counting engine and its C ABI.
+//
+// Implements the five reference entry points (include/corrfunc_theory.h):
+// option handling, bin setup, grid selection and the finalisation (autocorr
+// doubling, self pairs, averages, analytic RR) run on the host exactly as the
+// reference drivers do; gridding, the cell-pair work list and the pair counting
+// run on the GPU (cf_kernels.hip).
+#include <hip/hip_runtime.h>
+
+#include <ctype.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "cf_kernels.hip"
+#include "../../include/corrfunc_theory.h"
+#include "../../include/corrfunc_amd.h"
+
+namespace cfamd {
+
+#define CF_CHECK(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            fprintf(stderr, "corrfunc_amd: HIP error %s at %s:%d (%s)\n", hipGetErrorString(e_), \
+                    __FILE__, __LINE__, #call);                                                 \
+            throw HipFailure();                                                                 \
+        }                                                                                       \
+    } while (0)
+
+struct HipFailure {};
+struct InputError {};
+
+// --------------------------------------------------------------- state
+// Grow-only device buffers, reused across calls (the entry points are not
+// re-entrant, like the reference's).
+struct Arena {
+    std::unordered_map<std::string, std::pair<void*, size_t>> slots;
+    void* get(const std::string& name, size_t bytes)
+    {
+        if (bytes == 0) bytes = 256;
+        auto it = slots.find(name);
+        if (it != slots.end() && it->second.second >= bytes) return it->second.first;
+        if (it != slots.end()) {
+            CF_CHECK(hipFree(it->second.first));
+            slots.erase(it);
+        }
+        void* p = nullptr;
+        size_t cap = bytes + bytes / 8;
+        CF_CHECK(hipMalloc(&p, cap));
+        slots[name] = {p, cap};
+        return p;
+    }
+};
+
+struct Engine {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Arena arena;
+    int engine_kind = CF_AMD_ENGINE_TUNED;
+    cf_amd_stats last;
+    int waves_per_cu = 8;
+};
+
+static Engine& eng()
+{
+    static Engine E;
+    if (!E.init) {
+        CF_CHECK(hipGetDevice(&E.device));
+        CF_CHECK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+        CF_CHECK(hipEventCreate(&E.ev0));
+        CF_CHECK(hipEventCreate(&E.ev1));
+        const char* k = getenv("CORRFUNC_AMD_ENGINE");
+        if (k && strcmp(k, "plain") == 0) E.engine_kind = CF_AMD_ENGINE_PLAIN;
+        memset(&E.last, 0, sizeof E.last);
+        E.init = true;
+    }
+    return E;
+}
+
+static double now_s()
+{
+    struct timeval t;
+    gettimeofday(&t, NULL);
+    return t.tv_sec + 1e-6 * t.tv_usec;
+}
+
+// -------------------------------------------------- bins (utils.c:62-106)
+static int setup_bins(const char* fname, double* rmin, double* rmax, int* nbin, std::vector<double>& rupp)
+{
+    FILE* fp = fopen(fname, "rt");
+    if (!fp) {
+        fprintf(stderr, "Error: could not open bin file `%s'\n", fname);
+        return EXIT_FAILURE;
+    }
+    char line[10000];
+    int64_t nl = 0;
+    while (fgets(line, sizeof line, fp)) {
+        char* c = line;
+        while (*c != '\0' && isspace((unsigned char)*c)) c++;
+        if (*c != '\0' && *c != '#') nl++;
+    }
+    fclose(fp);
+    *nbin = (int)nl + 1;
+    rupp.assign((size_t)*nbin + 1, 0.0);
+    fp = fopen(fname, "r");
+    if (!fp) return EXIT_FAILURE;
+    char buf[1000];
+    int index = 1;
+    double lo, hi;
+    while (fgets(buf, sizeof buf, fp)) {
+        if (sscanf(buf, "%lf %lf", &lo, &hi) == 2) {
+            if (index == 1) {
+                *rmin = lo;
+                rupp[0] = lo;
+            }
+            if (index < *nbin + 1) rupp[index] = hi;
+            index++;
+        }
+    }
+    fclose(fp);
+    *rmax = rupp[index - 1 < *nbin ? index - 1 : *nbin];
+    rupp[*nbin] = *rmax;
+    rupp[*nbin - 1] = *rmax;
+    return EXIT_SUCCESS;
+}
+
+// ------------------------------------------------ input residency
+static bool is_device_ptr(const void* p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+// Returns a device pointer holding n elements of `esize` bytes from `p`
+// (copied when p is host memory).
+static const void* to_device(const std::string& slot, const void* p, int64_t n, size_t esize)
+{
+    if (!p) return nullptr;
+    if (is_device_ptr(p)) return p;
+    Engine& E = eng();
+    void* d = E.arena.get(slot, (size_t)n * esize);
+    CF_CHECK(hipMemcpyAsync(d, p, (size_t)n * esize, hipMemcpyHostToDevice, E.stream));
+    return d;
+}
+
+// ----------------------------------------------------------- helpers
+static void device_minmax(const void* X, const void* Y, const void* Z, int64_t n, size_t esize, double out[6])
+{
+    Engine& E = eng();
+    const int nb = 512;
+    double* part = (double*)E.arena.get("mm_part", sizeof(double) * 6 * nb);
+    double* res = (double*)E.arena.get("mm_res", sizeof(double) * 6);
+    if (esize == 4)
+        k_minmax<float><<<nb, 256, 0, E.stream>>>(n, (const float*)X, (const float*)Y, (const float*)Z, part);
+    else
+        k_minmax<double><<<nb, 256, 0, E.stream>>>(n, (const double*)X, (const double*)Y, (const double*)Z, part);
+    k_minmax_final<<<1, 64, 0, E.stream>>>(nb, part, res);
+    CF_CHECK(hipGetLastError());
+    CF_CHECK(hipMemcpyAsync(out, res, sizeof(double) * 6, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+}
+
+// exclusive scan of n int32 values; out has n+1 entries (out[n] = total)
+static void device_scan(const int* in, int* out, int64_t n, const std::string& tag, int depth = 0)
+{
+    Engine& E = eng();
+    if (n <= 0) {
+        CF_CHECK(hipMemsetAsync(out, 0, sizeof(int), E.stream));
+        return;
+    }
+    const int64_t per = (int64_t)kScanBlock * kScanItems;
+    const int64_t nblocks = (n + per - 1) / per;
+    int* bsum = (int*)E.arena.get(tag + "_bs" + std::to_string(depth), sizeof(int) * (nblocks + 1));
+    int* boff = (int*)E.arena.get(tag + "_bo" + std::to_string(depth), sizeof(int) * (nblocks + 1));
+    k_scan_blocks<<<(unsigned)nblocks, kScanBlock, 0, E.stream>>>(n, in, out, bsum);
+    CF_CHECK(hipGetLastError());
+    if (nblocks > 1) {
+        device_scan(bsum, boff, nblocks, tag, depth + 1);
+        k_scan_add<<<(unsigned)nblocks, kScanBlock, 0, E.stream>>>(n, out, boff);
+        CF_CHECK(hipGetLastError());
+        // total = boff[nblocks]
+        CF_CHECK(hipMemcpyAsync(out + n, boff + nblocks, sizeof(int), hipMemcpyDeviceToDevice, E.stream));
+    } else {
+        CF_CHECK(hipMemcpyAsync(out + n, bsum, sizeof(int), hipMemcpyDeviceToDevice, E.stream));
+    }
+}
+
+struct GridSpec {
+    int nx, ny, nz;
+    double xmin, ymin, zmin, xmax, ymax, zmax;
+    double xbs, ybs, zbs, xinv, yinv, zinv;
+};
+
+// gridlink_utils.c.src:31-49
+static int get_binsize(double xdiff, double xwrap, double rmax, int refine, int max_ncells, double* binsize, int* nl)
+{
+    int nmesh = (int)(refine * xdiff / rmax);
+    nmesh = nmesh < 1 ? 1 : nmesh;
+    if (xwrap > 0 && rmax >= xwrap / 2) {
+        fprintf(stderr, "Error: rmax=%f must be less than half of periodic boxize=%f to avoid double-counting particles\n",
+                rmax, xwrap);
+        return EXIT_FAILURE;
+    }
+    if (nmesh > max_ncells) nmesh = max_ncells;
+    if (nmesh < 2) nmesh = 2;
+    *binsize = xdiff / nmesh;
+    *nl = nmesh;
+    return EXIT_SUCCESS;
+}
+
+static int make_grid(GridSpec& g, const double ext[6], const double gsz[3], const double wrap[3], const int refine[3],
+                     int max_cells)
+{
+    g.xmin = ext[0]; g.ymin = ext[1]; g.zmin = ext[2];
+    g.xmax = ext[3]; g.ymax = ext[4]; g.zmax = ext[5];
+    if (get_binsize(g.xmax - g.xmin, wrap[0], gsz[0], refine[0], max_cells, &g.xbs, &g.nx) ||
+        get_binsize(g.ymax - g.ymin, wrap[1], gsz[1], refine[1], max_cells, &g.ybs, &g.ny) ||
+        get_binsize(g.zmax - g.zmin, wrap[2], gsz[2], refine[2], max_cells, &g.zbs, &g.nz))
+        return EXIT_FAILURE;
+    g.xinv = g.xbs > 0 ? 1.0 / g.xbs : 0.;
+    g.yinv = g.ybs > 0 ? 1.0 / g.ybs : 0.;
+    g.zinv = g.zbs > 0 ? 1.0 / g.zbs : 0.;
+    return EXIT_SUCCESS;
+}
+
+// Device counting sort of one particle set into the grid (gridlink_impl.c.src:65-436).
+static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n, const void* X, const void* Y,
+                           const void* Z, const void* W, size_t esize, DevLattice& L)
+{
+    Engine& E = eng();
+    L.nx = g.nx; L.ny = g.ny; L.nz = g.nz;
+    L.ncells = (int64_t)g.nx * g.ny * g.nz;
+    L.n = n;
+    L.xmin = g.xmin; L.ymin = g.ymin; L.zmin = g.zmin;
+    L.xbs = g.xbs; L.ybs = g.ybs; L.zbs = g.zbs;
+    L.xinv = g.xinv; L.yinv = g.yinv; L.zinv = g.zinv;
+    const int64_t nb = L.ncells * kSlabs;
+    if (nb >= (int64_t)INT32_MAX || n >= (int64_t)INT32_MAX) {
+        fprintf(stderr, "Error: grid of %lld cells or %lld particles exceeds the engine's 32-bit indexing\n",
+                (long long)L.ncells, (long long)n);
+        return EXIT_FAILURE;
+    }
+    int* counts = (int*)E.arena.get(tag + "_cnt", sizeof(int) * (nb + 1));
+    int* fill = (int*)E.arena.get(tag + "_fill", sizeof(int) * (nb + 1));
+    int* bucket = (int*)E.arena.get(tag + "_bucket", sizeof(int) * (n + 1));
+    unsigned long long* oob = (unsigned long long*)E.arena.get(tag + "_oob", 8);
+    L.bstart = (int*)E.arena.get(tag + "_bstart", sizeof(int) * (nb + 1));
+    L.x = (double*)E.arena.get(tag + "_x", sizeof(double) * n);
+    L.y = (double*)E.arena.get(tag + "_y", sizeof(double) * n);
+    L.z = (double*)E.arena.get(tag + "_z", sizeof(double) * n);
+    L.w = W ? (double*)E.arena.get(tag + "_w", sizeof(double) * n) : nullptr;
+    L.f4 = (float4*)E.arena.get(tag + "_f4", sizeof(float4) * n);
+    L.bounds = (double*)E.arena.get(tag + "_bounds", sizeof(double) * 6 * L.ncells);
+    CF_CHECK(hipMemsetAsync(counts, 0, sizeof(int) * (nb + 1), E.stream));
+    CF_CHECK(hipMemsetAsync(fill, 0, sizeof(int) * (nb + 1), E.stream));
+    CF_CHECK(hipMemsetAsync(oob, 0, 8, E.stream));
+    GridArgs ga{g.xmin, g.ymin, g.zmin, g.xinv, g.yinv, g.zinv, g.xbs, g.ybs, g.zbs, g.nx, g.ny, g.nz};
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (n > 0) {
+        if (esize == 4)
+            k_bucket_count<float><<<blocks, 256, 0, E.stream>>>(n, (const float*)X, (const float*)Y, (const float*)Z,
+                                                                 ga, bucket, counts, oob);
+        else
+            k_bucket_count<double><<<blocks, 256, 0, E.stream>>>(n, (const double*)X, (const double*)Y,
+                                                                  (const double*)Z, ga, bucket, counts, oob);
+        CF_CHECK(hipGetLastError());
+    }
+    unsigned long long h_oob = 0;
+    CF_CHECK(hipMemcpyAsync(&h_oob, oob, 8, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    if (h_oob) {
+        fprintf(stderr, "Error: %llu particles are out of bounds. Check periodic wrapping?\n", h_oob);
+        return EXIT_FAILURE;
+    }
+    device_scan(counts, L.bstart, nb, tag + "_scan");
+    if (n > 0) {
+        if (esize == 4)
+            k_scatter<float><<<blocks, 256, 0, E.stream>>>(n, (const float*)X, (const float*)Y, (const float*)Z,
+                                                            (const float*)W, ga, bucket, L.bstart, fill, L.x, L.y,
+                                                            L.z, L.w, L.f4);
+        else
+            k_scatter<double><<<blocks, 256, 0, E.stream>>>(n, (const double*)X, (const double*)Y, (const double*)Z,
+                                                             (const double*)W, ga, bucket, L.bstart, fill, L.x,
+                                                             L.y, L.z, L.w, L.f4);
+        CF_CHECK(hipGetLastError());
+    }
+    k_cell_bounds<<<(unsigned)((L.ncells + 255) / 256), 256, 0, E.stream>>>(L.ncells, L.bstart, L.x, L.y, L.z,
+                                                                           L.bounds);
+    CF_CHECK(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
+// Device work list of neighbour-cell pairs (gridlink_impl.c.src:439-625).
+static int cellpairs_device(const DevLattice& L1, const DevLattice& L2, const int refine[3], const int per[3],
+                            const double wrap[3], double max3, double max2, double max1, int min_sep_opt,
+                            int autocorr, CellPairRec** out, int* ncp_out, double* tested)
+{
+    Engine& E = eng();
+    NgbArgs a;
+    a.nx = L1.nx; a.ny = L1.ny; a.nz = L1.nz;
+    a.rx = refine[0]; a.ry = refine[1]; a.rz = refine[2];
+    a.px = per[0]; a.py = per[1]; a.pz = per[2];
+    a.autocorr = autocorr;
+    a.min_sep_opt = min_sep_opt;
+    a.dupcheck = (per[0] || per[1] || per[2]) &&
+                 (L1.nx < 2 * refine[0] + 1 || L1.ny < 2 * refine[1] + 1 || L1.nz < 2 * refine[2] + 1);
+    a.xwrap = wrap[0]; a.ywrap = wrap[1]; a.zwrap = wrap[2];
+    a.max3 = max3; a.max2 = max2; a.max1 = max1;
+    const int nngb = (2 * refine[0] + 1) * (2 * refine[1] + 1) * (2 * refine[2] + 1);
+    const int64_t total = L1.ncells * nngb;
+    if (total >= (int64_t)INT32_MAX) {
+        fprintf(stderr, "Error: %lld candidate cell pairs exceed the engine's 32-bit indexing\n", (long long)total);
+        return EXIT_FAILURE;
+    }
+    int* flags = (int*)E.arena.get("cp_flags", sizeof(int) * (total + 1));
+    int* pos = (int*)E.arena.get("cp_pos", sizeof(int) * (total + 1));
+    double* dtested = (double*)E.arena.get("cp_tested", sizeof(double));
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    k_cellpair_flag<<<blocks, 256, 0, E.stream>>>(a, L1.ncells, nngb, L1.bstart, L2.bstart, L1.bounds, L2.bounds,
+                                                  flags);
+    CF_CHECK(hipGetLastError());
+    device_scan(flags, pos, total, "cp_scan");
+    int ncp = 0;
+    CF_CHECK(hipMemcpyAsync(&ncp, pos + total, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    CellPairRec* cps = (CellPairRec*)E.arena.get("cp_list", sizeof(CellPairRec) * (ncp + 1));
+    CF_CHECK(hipMemsetAsync(dtested, 0, sizeof(double), E.stream));
+    k_cellpair_emit<<<blocks, 256, 0, E.stream>>>(a, L1.ncells, nngb, L1.bstart, L2.bstart, L1.bounds, L2.bounds,
+                                                  pos, cps, dtested);
+    CF_CHECK(hipGetLastError());
+    CF_CHECK(hipMemcpyAsync(tested, dtested, sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    *out = cps;
+    *ncp_out = ncp;
+    return EXIT_SUCCESS;
+}
+
+// ------------------------------------------------------------- driver
+struct CallSpec {
+    int mode;
+    int64_t N1, N2;
+    void *X1, *Y1, *Z1, *X2, *Y2, *Z2;
+    int autocorr;
+    const char* binfile;
+    double pimax;       // rp_pi / wp
+    double mu_max;      // s_mu
+    int nmu_bins;
+    double boxsize;     // xi / wp
+    int numthreads;
+};
+
+struct CallOut {
+    std::vector<double> rupp;
+    int nbin = 0, n2 = 0;
+    std::vector<uint64_t> np;
+    std::vector<double> avg, wavg;
+    double wsum = 0, wsq = 0;  // sum of weights / of squared weights over D1
+    int need_w = 0;
+};
+
+static int resolve_isa(int isa_in)
+{
+    if (isa_in >= 0) return isa_in;
+#if defined(__x86_64__) && !defined(__HIP_DEVICE_COMPILE__)
+    __builtin_cpu_init();
+    if (__builtin_cpu_supports("avx512f")) return AVX512F;
+    if (__builtin_cpu_supports("avx2")) return AVX2;
+    if (__builtin_cpu_supports("avx")) return AVX;
+    if (__builtin_cpu_supports("sse4.2")) return SSE42;
+#endif
+    return FALLBACK;
+}
+
+static int r2form_for(int mode, int isa_id)
+{
+    switch (mode) {
+    case MODE_DD: return isa_id == AVX512F ? R2_FMA_DZ_DY_DX : R2_SUM_XY_Z;
+    case MODE_XI: return isa_id == AVX512F ? R2_FMA_DZ_DY_DX : ((isa_id == AVX || isa_id == AVX2) ? R2_X_SUM_YZ : R2_SUM_XY_Z);
+    case MODE_RPPI: return isa_id == AVX512F ? RP_FMA : RP_SUM;
+    case MODE_WP: return (isa_id == AVX512F || isa_id == AVX2) ? RP_FMA : RP_SUM;
+    case MODE_SMU: return isa_id == AVX512F ? R2_FMA_DX_DY_DZ : R2_SUM_XY_Z;
+    }
+    return R2_SUM_XY_Z;
+}
+
+static inline float f32_up(double v) { return nextafterf((float)v, INFINITY); }
+static inline float f32_down(double v) { return nextafterf((float)v, -INFINITY); }
+
+static int run_core(const CallSpec& c, struct config_options* o, struct extra_options* extra, CallOut& R)
+{
+    Engine& E = eng();
+    cf_amd_stats st;
+    memset(&st, 0, sizeof st);
+    const double t_start = now_s();
+    const int mode = c.mode;
+    const int is2d = mode == MODE_RPPI || mode == MODE_WP;
+    const bool xiwp = mode == MODE_XI || mode == MODE_WP;
+    const size_t esize = o->float_type;
+    const int need_w = extra->weight_method != NONE;
+    R.need_w = need_w;
+    if (need_w && extra->weight_method != PAIR_PRODUCT) {
+        fprintf(stderr, "Error: unknown weight method %d\n", (int)extra->weight_method);
+        return EXIT_FAILURE;
+    }
+    if (need_w && (extra->weights0.weights[0] == nullptr || (!c.autocorr && extra->weights1.weights[0] == nullptr))) {
+        fprintf(stderr, "Error: weight_method requires weights arrays\n");
+        return EXIT_FAILURE;
+    }
+    // ---- bins
+    double rmin = 0, rmax = 0;
+    int nbin = 0;
+    if (setup_bins(c.binfile, &rmin, &rmax, &nbin, R.rupp) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (!(rmin >= 0.0 && rmax > 0.0 && rmin < rmax && nbin > 0)) {
+        fprintf(stderr, "Error: Could not setup with R bins correctly. (rmin = %lf, rmax = %lf, with nbins = %d). "
+                        "Expected non-zero rmin/rmax with rmax > rmin and nbins >=1 \n", rmin, rmax, nbin);
+        return EXIT_FAILURE;
+    }
+    R.nbin = nbin;
+    if (mode == MODE_SMU) {
+        if (c.mu_max <= 0.0 || c.mu_max > 1.0) {
+            fprintf(stderr, "Error: max_mu = %lf. Please change it to be > 0 and <= 1.0\n", c.mu_max);
+            return EXIT_FAILURE;
+        }
+        if (c.nmu_bins < 1) {
+            fprintf(stderr, "Error: Number of mu bins = %d must be at least 1\n", c.nmu_bins);
+            return EXIT_FAILURE;
+        }
+    }
+    int n2 = 0;
+    double pimax = 0, inv_d2 = 0, sqr_mu_max = 0;
+    if (mode == MODE_RPPI) {
+        n2 = (int)c.pimax;
+        pimax = c.pimax;
+        const double dpi = pimax / n2;
+        inv_d2 = 1.0 / dpi;
+    } else if (mode == MODE_WP) {
+        pimax = c.pimax;
+    } else if (mode == MODE_SMU) {
+        n2 = c.nmu_bins;
+        pimax = rmax * c.mu_max;
+        const double dmu = c.mu_max / n2;
+        inv_d2 = 1.0 / dmu;
+        sqr_mu_max = c.mu_max * c.mu_max;
+    } else {
+        pimax = rmax;
+    }
+    R.n2 = n2;
+    const int64_t nslots = (mode == MODE_RPPI || mode == MODE_SMU) ? (int64_t)(n2 + 1) * (nbin + 1) : nbin;
+    std::vector<double> rupp_sqr(nbin);
+    for (int i = 0; i < nbin; i++) rupp_sqr[i] = R.rupp[i] * R.rupp[i];
+    int sorted = 1;
+    for (int i = 1; i + 1 < nbin; i++)
+        if (!(rupp_sqr[i] >= rupp_sqr[i - 1])) sorted = 0;
+
+    // ---- inputs in HBM
+    const void *X1 = to_device("in_x1", c.X1, c.N1, esize), *Y1 = to_device("in_y1", c.Y1, c.N1, esize),
+               *Z1 = to_device("in_z1", c.Z1, c.N1, esize);
+    const void *W1 = need_w ? to_device("in_w1", extra->weights0.weights[0], c.N1, esize) : nullptr;
+    const void *X2 = nullptr, *Y2 = nullptr, *Z2 = nullptr, *W2 = nullptr;
+    if (!c.autocorr) {
+        X2 = to_device("in_x2", c.X2, c.N2, esize);
+        Y2 = to_device("in_y2", c.Y2, c.N2, esize);
+        Z2 = to_device("in_z2", c.Z2, c.N2, esize);
+        W2 = need_w ? to_device("in_w2", extra->weights1.weights[0], c.N2, esize) : nullptr;
+    }
+
+    // ---- extent and wraps
+    double ext[6], wrap[3];
+    int per[3];
+    if (xiwp) {
+        ext[0] = ext[1] = ext[2] = 0.0;
+        ext[3] = ext[4] = ext[5] = c.boxsize;
+        wrap[0] = wrap[1] = wrap[2] = c.boxsize;
+        per[0] = per[1] = per[2] = 1;
+    } else {
+        double e1[6];
+        device_minmax(X1, Y1, Z1, c.N1, esize, e1);
+        memcpy(ext, e1, sizeof ext);
+        if (!c.autocorr) {
+            double e2[6];
+            device_minmax(X2, Y2, Z2, c.N2, esize, e2);
+            for (int d = 0; d < 3; d++) {
+                ext[d] = e2[d] < ext[d] ? e2[d] : ext[d];
+                ext[3 + d] = e2[3 + d] > ext[3 + d] ? e2[3 + d] : ext[3 + d];
+            }
+        }
+        if (o->periodic && o->boxsize == BOXSIZE_NOTGIVEN) {
+            fprintf(stderr, "boxsize = %g must be specified with periodic wrap. Please specify a non-zero boxsize, or "
+                            "zero to detect the particle extent, or -1 to make a dimension non-periodic.\n", o->boxsize);
+            return EXIT_FAILURE;
+        }
+        const double by = o->boxsize_y == BOXSIZE_NOTGIVEN ? o->boxsize : o->boxsize_y;
+        const double bz = o->boxsize_z == BOXSIZE_NOTGIVEN ? o->boxsize : o->boxsize_z;
+        per[0] = o->periodic && o->boxsize_x >= 0;
+        per[1] = o->periodic && by >= 0;
+        per[2] = o->periodic && bz >= 0;
+        wrap[0] = per[0] ? (o->boxsize_x > 0 ? o->boxsize_x : (ext[3] - ext[0])) : 0.;
+        wrap[1] = per[1] ? (by > 0 ? by : (ext[4] - ext[1])) : 0.;
+        wrap[2] = per[2] ? (bz > 0 ? bz : (ext[5] - ext[2])) : 0.;
+    }
+
+    // ---- refine heuristics (countpairs_impl.c.src:272-282 and per-mode variants)
+    if (get_bin_refine_scheme(o) == BINNING_DFL) {
+        if (mode == MODE_DD) {
+            for (int d = 0; d < 3; d++)
+                if (rmax < 0.05 * wrap[d]) o->bin_refine_factors[d] = 1;
+        } else if (mode == MODE_RPPI) {
+            if (rmax < 0.05 * wrap[0]) o->bin_refine_factors[0] = 1;
+            if (rmax < 0.05 * wrap[1]) o->bin_refine_factors[1] = 1;
+            if (pimax < 0.05 * wrap[2]) o->bin_refine_factors[2] = 1;
+        } else if (mode == MODE_XI) {
+            if (rmax < 0.05 * c.boxsize)
+                for (int d = 0; d < 3; d++) o->bin_refine_factors[d] = 1;
+        } else if (mode == MODE_WP) {
+            if (rmax < 0.05 * c.boxsize) o->bin_refine_factors[0] = o->bin_refine_factors[1] = 1;
+            if (pimax < 0.05 * c.boxsize) o->bin_refine_factors[2] = 1;
+        }
+    }
+    double gsz[3] = {rmax, rmax, rmax};
+    if (is2d || mode == MODE_SMU) gsz[2] = pimax;
+    int refine[3] = {o->bin_refine_factors[0], o->bin_refine_factors[1], o->bin_refine_factors[2]};
+    GridSpec g;
+    if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+    // boost (countpairs_impl.c.src:298-332; DDsmu variant s_mu_impl:313-340)
+    if (mode == MODE_SMU) {
+        if (g.nx <= 10 && g.ny <= 10 && g.nz <= 10 && o->max_cells_per_dim >= 10 &&
+            get_bin_refine_scheme(o) == BINNING_DFL) {
+            for (int d = 0; d < 3; d++) o->bin_refine_factors[d] *= 1;
+        }
+    } else {
+        const double avg_np = ((double)c.N1) / ((double)g.nx * g.ny * g.nz);
+        const int max_nmesh = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
+        if ((max_nmesh <= 10 || avg_np >= 250) && max_nmesh < o->max_cells_per_dim &&
+            get_bin_refine_scheme(o) == BINNING_DFL) {
+            o->bin_refine_factors[0] += 1;
+            o->bin_refine_factors[1] += 1;
+        }
+    }
+    refine[0] = o->bin_refine_factors[0];
+    refine[1] = o->bin_refine_factors[1];
+    refine[2] = o->bin_refine_factors[2];
+    if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+
+    // ---- gridding on the GPU
+    DevLattice L1, L2;
+    if (gridlink_device("l1", g, c.N1, X1, Y1, Z1, W1, esize, L1) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (!c.autocorr) {
+        if (gridlink_device("l2", g, c.N2, X2, Y2, Z2, W2, esize, L2) != EXIT_SUCCESS) return EXIT_FAILURE;
+    } else {
+        L2 = L1;
+    }
+    const double t_grid = now_s();
+
+    // ---- cell pairs
+    double max3 = -1, max2 = -1, max1 = -1;
+    if (mode == MODE_DD || mode == MODE_XI) max3 = rmax;
+    else if (is2d) { max2 = rmax; max1 = pimax; }
+    else { max3 = rmax; max1 = pimax; }
+    CellPairRec* cps = nullptr;
+    int ncp = 0;
+    double tested = 0;
+    if (cellpairs_device(L1, L2, refine, per, wrap, max3, max2, max1, o->enable_min_sep_opt, c.autocorr, &cps, &ncp,
+                         &tested) != EXIT_SUCCESS)
+        return EXIT_FAILURE;
+
+    // ---- launch parameters
+    PairParams P;
+    memset(&P, 0, sizeof P);
+    const int isa_id = resolve_isa(o->instruction_set);
+    P.mode = mode;
+    P.r2form = r2form_for(mode, isa_id);
+    P.float_index = (isa_id == AVX512F) && (mode == MODE_RPPI || mode == MODE_SMU);
+    P.nbin = nbin;
+    P.n2 = n2;
+    P.nslots = (int)nslots;
+    P.need_avg = o->need_avg_sep;
+    P.need_w = need_w;
+    P.autocorr = c.autocorr;
+    P.sorted_edges = sorted;
+    P.sqr_rmax = rupp_sqr[nbin - 1];
+    P.sqr_rmin = rupp_sqr[0];
+    P.pimax = pimax;
+    P.inv_d2 = inv_d2;
+    P.sqr_mu_max = sqr_mu_max;
+    if (mode == MODE_SMU) {
+        const double a = sqrt(P.sqr_rmax - 0.0 - 0.0);
+        P.max_all_dz_same = pimax < a ? pimax : a;
+    } else {
+        P.max_all_dz_same = sqrt(P.sqr_rmax - 0.0 - 0.0);
+    }
+    P.xwrap = wrap[0];
+    P.ywrap = wrap[1];
+    P.zwrap = wrap[2];
+    // Edge windows: the reference's pruning tests can only reject pairs whose
+    // separation lies within a few ulps of the coordinate scale below rmax / pimax.
+    const double coord_scale = fabs(ext[0]) + fabs(ext[1]) + fabs(ext[2]) + fabs(ext[3]) + fabs(ext[4]) +
+                               fabs(ext[5]) + wrap[0] + wrap[1] + wrap[2] + rmax;
+    P.near_r2 = 1e-9 * P.sqr_rmax + 1e-9 * rmax * coord_scale;
+    P.near_dz = 1e-9 * pimax + 1e-9 * coord_scale;
+    // FP32 filter margin: |dx_f32 - dx| <= eps for cell-local coordinates.
+    {
+        const double cs = fmax(g.xbs, fmax(g.ybs, g.zbs));
+        const int rmaxr = refine[0] > refine[1] ? (refine[0] > refine[2] ? refine[0] : refine[2])
+                                                : (refine[1] > refine[2] ? refine[1] : refine[2]);
+        const double B = (rmaxr + 3) * cs + rmax + pimax;
+        const double eps = 8.0 * B * 0x1p-24;
+        const double r_hi = sqrt(P.sqr_rmax), r_lo = sqrt(P.sqr_rmin);
+        const double m_hi = 2.0 * sqrt(3.0) * r_hi * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmax;
+        const double m_lo = 2.0 * sqrt(3.0) * r_lo * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmin;
+        P.hi2_f = f32_up(P.sqr_rmax + m_hi);
+        P.lo2_f = P.sqr_rmin - m_lo > 0 ? f32_down(P.sqr_rmin - m_lo) : -INFINITY;
+        P.zreach_f = f32_up(pimax + 4.0 * eps);
+        P.zreach = (is2d || mode == MODE_SMU) ? pimax : sqrt(P.sqr_rmax);
+        P.slab_margin = 1e-7 * (coord_scale + g.zbs);
+    }
+
+    // ---- histograms
+    unsigned long long* g_np = (unsigned long long*)E.arena.get("h_np", sizeof(uint64_t) * nslots);
+    double* g_avg = (double*)E.arena.get("h_avg", sizeof(double) * nslots);
+    double* g_w = (double*)E.arena.get("h_w", sizeof(double) * nslots);
+    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * nbin);
+    CF_CHECK(hipMemsetAsync(g_np, 0, sizeof(uint64_t) * nslots, E.stream));
+    CF_CHECK(hipMemsetAsync(g_avg, 0, sizeof(double) * nslots, E.stream));
+    CF_CHECK(hipMemsetAsync(g_w, 0, sizeof(double) * nslots, E.stream));
+    CF_CHECK(hipMemcpyAsync(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice, E.stream));
+    const size_t lds_hist = (size_t)nslots * 24 + (size_t)nbin * 8;
+    if (lds_hist + 4096 > 64 * 1024) {
+        fprintf(stderr, "Error: %lld histogram slots exceed the engine's LDS histogram capacity\n", (long long)nslots);
+        return EXIT_FAILURE;
+    }
+
+    int nitems = 0;
+    CF_CHECK(hipEventRecord(E.ev0, E.stream));
+    if (ncp > 0) {
+        if (E.engine_kind == CF_AMD_ENGINE_PLAIN) {
+            k_pairs_plain<<<ncp, 128, lds_hist, E.stream>>>(P, L1, L2, cps, d_edges, g_np, g_avg, g_w);
+            CF_CHECK(hipGetLastError());
+        } else {
+            int* nch = (int*)E.arena.get("wi_n", sizeof(int) * (ncp + 1));
+            int* off = (int*)E.arena.get("wi_off", sizeof(int) * (ncp + 1));
+            k_count_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, L1.bstart, nch);
+            CF_CHECK(hipGetLastError());
+            device_scan(nch, off, ncp, "wi_scan");
+            CF_CHECK(hipMemcpyAsync(&nitems, off + ncp, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+            CF_CHECK(hipStreamSynchronize(E.stream));
+            WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * (nitems + 1));
+            k_emit_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, L1.bstart, off, items);
+            CF_CHECK(hipGetLastError());
+            int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
+            CF_CHECK(hipMemsetAsync(counter, 0, sizeof(int), E.stream));
+            const int threads = 256;
+            const size_t lds = lds_hist + (size_t)(threads / 64) * kQueue * 5;
+            int ncu = 256;
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, E.device) == hipSuccess) ncu = prop.multiProcessorCount;
+            int per_cu = (int)((160 * 1024) / (lds + 1024));
+            per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+            long blocks = (long)ncu * per_cu;
+            const long needed = (nitems + 3) / 4;
+            if (blocks > needed) blocks = needed > 0 ? needed : 1;
+            if (is2d)
+                k_pairs_tuned<1><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
+                                                                               d_edges, g_np, g_avg, g_w);
+            else if (mode == MODE_SMU)
+                k_pairs_tuned<2><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
+                                                                               d_edges, g_np, g_avg, g_w);
+            else
+                k_pairs_tuned<0><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
+                                                                               d_edges, g_np, g_avg, g_w);
+            CF_CHECK(hipGetLastError());
+        }
+    }
+    CF_CHECK(hipEventRecord(E.ev1, E.stream));
+    R.np.assign(nslots, 0);
+    R.avg.assign(nslots, 0.0);
+    R.wavg.assign(nslots, 0.0);
+    CF_CHECK(hipMemcpyAsync(R.np.data(), g_np, sizeof(uint64_t) * nslots, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(R.avg.data(), g_avg, sizeof(double) * nslots, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(R.wavg.data(), g_w, sizeof(double) * nslots, hipMemcpyDeviceToHost, E.stream));
+    if (need_w) {
+        double* ws = (double*)E.arena.get("wsums", sizeof(double) * 2);
+        CF_CHECK(hipMemsetAsync(ws, 0, sizeof(double) * 2, E.stream));
+        if (c.N1 > 0) k_weight_sums<<<256, 256, 0, E.stream>>>(c.N1, L1.w, ws);
+        CF_CHECK(hipGetLastError());
+        double h[2];
+        CF_CHECK(hipMemcpyAsync(h, ws, sizeof h, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        R.wsum = h[0];
+        R.wsq = h[1];
+    }
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    float kms = 0;
+    CF_CHECK(hipEventElapsedTime(&kms, E.ev0, E.ev1));
+
+    st.nmesh[0] = g.nx; st.nmesh[1] = g.ny; st.nmesh[2] = g.nz;
+    st.refine[0] = refine[0]; st.refine[1] = refine[1]; st.refine[2] = refine[2];
+    st.num_cell_pairs = ncp;
+    st.num_work_items = nitems;
+    st.tested_pairs = tested;
+    st.kernel_ms = kms;
+    st.grid_ms = (t_grid - t_start) * 1e3;
+    st.total_ms = (now_s() - t_start) * 1e3;
+    st.engine = E.engine_kind;
+    E.last = st;
+    return EXIT_SUCCESS;
+}
+
+static int check_common(const struct config_options* o)
+{
+    if (!(o->float_type == sizeof(float) || o->float_type == sizeof(double))) {
+        fprintf(stderr, "ERROR: In %s> Can only handle doubles or floats. Got an array of size = %zu\n", "corrfunc_amd",
+                o->float_type);
+        return EXIT_FAILURE;
+    }
+    if (strncmp(o->version, CORRFUNC_API_VERSION, sizeof(o->version) - 1) != 0 &&
+        strncmp(o->version, "2.5.3", sizeof(o->version) - 1) != 0) {
+        fprintf(stderr, "Error: Do not know this API version = `%s'. Expected version = `%s'\n", o->version,
+                CORRFUNC_API_VERSION);
+        return EXIT_FAILURE;
+    }
+    return EXIT_SUCCESS;
+}
+
+static void fix_refine(struct config_options* o)
+{
+    for (int i = 0; i < 3; i++) {
+        if (o->bin_refine_factors[i] < 1) {
+            fprintf(stderr, "Warning: bin refine factor along axis = %d *must* be >=1. Instead found bin refine factor =%d\n",
+                    i, o->bin_refine_factors[i]);
+            reset_bin_refine_factors(o);
+            break;
+        }
+    }
+}
+
+static void fix_maxcells(struct config_options* o)
+{
+    if (o->max_cells_per_dim == 0) {
+        fprintf(stderr, "Warning: Max. cells per dimension is set to 0 - resetting to `NLATMAX' = %d\n", NLATMAX);
+        o->max_cells_per_dim = NLATMAX;
+    }
+}
+
+// autocorr doubling, self pairs, averages (countpairs_impl.c.src:608-664)
+static void finalize(CallOut& R, int autocorr, int64_t ND1, int need_avg, int self_slot)
+{
+    const size_t nb = R.np.size();
+    if (autocorr == 1) {
+        for (size_t i = 0; i < nb; i++) {
+            R.np[i] *= 2;
+            if (need_avg) R.avg[i] *= 2.0;
+            if (R.need_w) R.wavg[i] *= 2.0;
+        }
+        if (R.rupp[0] <= 0.0) {
+            R.np[self_slot] += (uint64_t)ND1;
+            if (R.need_w) R.wavg[1] += R.wsq;
+        }
+    }
+    for (size_t i = 0; i < nb; i++) {
+        if (R.np[i] > 0) {
+            if (need_avg) R.avg[i] /= (double)R.np[i];
+            if (R.need_w) R.wavg[i] /= (double)R.np[i];
+        }
+    }
+}
+
+template <typename F>
+static int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const HipFailure&) {
+        return EXIT_FAILURE;
+    } catch (const InputError&) {
+        return EXIT_FAILURE;
+    }
+}
+
+}  // namespace cfamd
+
+using namespace cfamd;
+
+// ===================================================================== C ABI
+extern "C" {
+
+void free_results(results_countpairs* r)
+{
+    if (!r) return;
+    free(r->rupp); r->rupp = NULL;
+    free(r->npairs); r->npairs = NULL;
+    free(r->rpavg); r->rpavg = NULL;
+    free(r->weightavg); r->weightavg = NULL;
+    r->nbin = 0;
+}
+
+void free_results_rp_pi(results_countpairs_rp_pi* r)
+{
+    if (!r) return;
+    free(r->rupp); r->rupp = NULL;
+    free(r->npairs); r->npairs = NULL;
+    free(r->rpavg); r->rpavg = NULL;
+    free(r->weightavg); r->weightavg = NULL;
+    r->nbin = 0;
+    r->npibin = 0;
+}
+
+void free_results_s_mu(results_countpairs_s_mu* r)
+{
+    if (!r) return;
+    free(r->supp); r->supp = NULL;
+    free(r->npairs); r->npairs = NULL;
+    free(r->savg); r->savg = NULL;
+    free(r->weightavg); r->weightavg = NULL;
+    r->nsbin = 0;
+    r->nmu_bins = 0;
+}
+
+void free_results_xi(results_countpairs_xi* r)
+{
+    if (!r) return;
+    free(r->rupp); r->rupp = NULL;
+    free(r->xi); r->xi = NULL;
+    free(r->npairs); r->npairs = NULL;
+    free(r->ravg); r->ravg = NULL;
+    free(r->weightavg); r->weightavg = NULL;
+    r->nbin = 0;
+}
+
+void free_results_wp(results_countpairs_wp* r)
+{
+    if (!r) return;
+    free(r->rupp); r->rupp = NULL;
+    free(r->wp); r->wp = NULL;
+    free(r->npairs); r->npairs = NULL;
+    free(r->rpavg); r->rpavg = NULL;
+    free(r->weightavg); r->weightavg = NULL;
+    r->nbin = 0;
+}
+
+int countpairs(const int64_t ND1, void* X1, void* Y1, void* Z1, const int64_t ND2, void* X2, void* Y2, void* Z2,
+               const int numthreads, const int autocorr, const char* binfile, results_countpairs* results,
+               struct config_options* options, struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        const double t0 = now_s();
+        fix_maxcells(options);
+        fix_refine(options);
+        options->sort_on_z = 1;
+        CallSpec c{MODE_DD, ND1, ND2, X1, Y1, Z1, X2, Y2, Z2, autocorr, binfile, 0, 0, 0, 0, numthreads};
+        CallOut R;
+        if (run_core(c, options, extra, R) != EXIT_SUCCESS) return EXIT_FAILURE;
+        finalize(R, autocorr, ND1, options->need_avg_sep, 1);
+        const int nbin = R.nbin;
+        results->nbin = nbin;
+        results->npairs = (uint64_t*)malloc(sizeof(uint64_t) * nbin);
+        results->rupp = (double*)malloc(sizeof(double) * nbin);
+        results->rpavg = (double*)calloc(nbin, sizeof(double));
+        results->weightavg = (double*)calloc(nbin, sizeof(double));
+        if (!results->npairs || !results->rupp || !results->rpavg || !results->weightavg) {
+            free_results(results);
+            return EXIT_FAILURE;
+        }
+        for (int i = 0; i < nbin; i++) {
+            results->npairs[i] = R.np[i];
+            results->rupp[i] = R.rupp[i];
+            results->rpavg[i] = options->need_avg_sep ? R.avg[i] : 0.0;
+            results->weightavg[i] = R.need_w ? R.wavg[i] : 0.0;
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int countpairs_rp_pi(const int64_t ND1, void* X1, void* Y1, void* Z1, const int64_t ND2, void* X2, void* Y2, void* Z2,
+                     const int numthreads, const int autocorr, const char* binfile, const double pimax,
+                     results_countpairs_rp_pi* results, struct config_options* options, struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        const double t0 = now_s();
+        options->sort_on_z = 1;
+        fix_refine(options);
+        fix_maxcells(options);
+        CallSpec c{MODE_RPPI, ND1, ND2, X1, Y1, Z1, X2, Y2, Z2, autocorr, binfile, pimax, 0, 0, 0, numthreads};
+        CallOut R;
+        if (run_core(c, options, extra, R) != EXIT_SUCCESS) return EXIT_FAILURE;
+        const int npibin = R.n2;
+        finalize(R, autocorr, ND1, options->need_avg_sep, npibin + 1);
+        const int nbin = R.nbin;
+        const int64_t tot = (int64_t)(npibin + 1) * (nbin + 1);
+        results->nbin = nbin;
+        results->npibin = npibin;
+        results->pimax = pimax;
+        results->npairs = (uint64_t*)calloc(tot, sizeof(uint64_t));
+        results->rupp = (double*)malloc(sizeof(double) * nbin);
+        results->rpavg = (double*)calloc(tot, sizeof(double));
+        results->weightavg = (double*)calloc(tot, sizeof(double));
+        if (!results->npairs || !results->rupp || !results->rpavg || !results->weightavg) {
+            free_results_rp_pi(results);
+            return EXIT_FAILURE;
+        }
+        for (int i = 0; i < nbin; i++) {
+            results->rupp[i] = R.rupp[i];
+            for (int j = 0; j < npibin; j++) {
+                const int64_t idx = (int64_t)i * (npibin + 1) + j;
+                results->npairs[idx] = R.np[idx];
+                results->rpavg[idx] = options->need_avg_sep ? R.avg[idx] : 0.0;
+                results->weightavg[idx] = R.need_w ? R.wavg[idx] : 0.0;
+            }
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int countpairs_s_mu(const int64_t ND1, void* X1, void* Y1, void* Z1, const int64_t ND2, void* X2, void* Y2, void* Z2,
+                    const int numthreads, const int autocorr, const char* sbinfile, const double mu_max,
+                    const int nmu_bins, results_countpairs_s_mu* results, struct config_options* options,
+                    struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        const double t0 = now_s();
+        options->sort_on_z = 1;
+        fix_refine(options);
+        fix_maxcells(options);
+        if (options->fast_divide_and_NR_steps >= MAX_FAST_DIVIDE_NR_STEPS) options->fast_divide_and_NR_steps = 0;
+        CallSpec c{MODE_SMU, ND1, ND2, X1, Y1, Z1, X2, Y2, Z2, autocorr, sbinfile, 0, mu_max, nmu_bins, 0, numthreads};
+        CallOut R;
+        if (run_core(c, options, extra, R) != EXIT_SUCCESS) return EXIT_FAILURE;
+        finalize(R, autocorr, ND1, options->need_avg_sep, nmu_bins + 1);
+        const int nsbin = R.nbin;
+        const int64_t tot = (int64_t)(nmu_bins + 1) * (nsbin + 1);
+        results->nsbin = nsbin;
+        results->nmu_bins = nmu_bins;
+        results->mu_max = mu_max;
+        results->mu_min = 0.0;
+        results->npairs = (uint64_t*)calloc(tot, sizeof(uint64_t));
+        results->supp = (double*)malloc(sizeof(double) * nsbin);
+        results->savg = (double*)calloc(tot, sizeof(double));
+        results->weightavg = (double*)calloc(tot, sizeof(double));
+        if (!results->npairs || !results->supp || !results->savg || !results->weightavg) {
+            free_results_s_mu(results);
+            return EXIT_FAILURE;
+        }
+        for (int i = 0; i < nsbin; i++) {
+            results->supp[i] = R.rupp[i];
+            for (int j = 0; j < nmu_bins; j++) {
+                const int64_t idx = (int64_t)i * (nmu_bins + 1) + j;
+                results->npairs[idx] = R.np[idx];
+                results->savg[idx] = options->need_avg_sep ? R.avg[idx] : 0.0;
+                results->weightavg[idx] = R.need_w ? R.wavg[idx] : 0.0;
+            }
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+static int xi_wp_run(int mode, const int64_t ND, void* X, void* Y, void* Z, double boxsize, int numthreads,
+                     const char* binfile, double pimax, struct config_options* options, struct extra_options*& extra,
+                     struct extra_options* dummy, CallOut& R)
+{
+    if (extra == NULL) { *dummy = get_extra_options(NONE); extra = dummy; }
+    options->periodic = 1;
+    options->autocorr = 1;
+    options->sort_on_z = 1;
+    fix_maxcells(options);
+    fix_refine(options);
+    CallSpec c{mode, ND, 0, X, Y, Z, nullptr, nullptr, nullptr, 1, binfile, pimax, 0, 0, boxsize, numthreads};
+    if (run_core(c, options, extra, R) != EXIT_SUCCESS) return EXIT_FAILURE;
+    finalize(R, 1, ND, options->need_avg_sep, 1);
+    return EXIT_SUCCESS;
+}
+
+int countpairs_xi(const int64_t ND1, void* X1, void* Y1, void* Z1, const double boxsize, const int numthreads,
+                  const char* binfile, results_countpairs_xi* results, struct config_options* options,
+                  struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        const double t0 = now_s();
+        struct extra_options dummy;
+        CallOut R;
+        if (xi_wp_run(MODE_XI, ND1, X1, Y1, Z1, boxsize, numthreads, binfile, 0, options, extra, &dummy, R))
+            return EXIT_FAILURE;
+        const int nbin = R.nbin;
+        results->nbin = nbin;
+        results->npairs = (uint64_t*)malloc(sizeof(uint64_t) * nbin);
+        results->xi = (double*)malloc(sizeof(double) * nbin);
+        results->rupp = (double*)malloc(sizeof(double) * nbin);
+        results->ravg = (double*)malloc(sizeof(double) * nbin);
+        results->weightavg = (double*)calloc(nbin, sizeof(double));
+        if (!results->npairs || !results->xi || !results->rupp || !results->ravg || !results->weightavg) {
+            free_results_xi(results);
+            return EXIT_FAILURE;
+        }
+        // analytic RR (countpairs_xi_impl.c.src:581-628)
+        double wsum = (double)ND1, wsq = (double)ND1;
+        if (R.need_w) {
+            wsum = R.wsum;
+            wsq = (double)ND1 + R.wsq;
+        }
+        const double prefac = wsum * (wsum - wsum / ND1) / (boxsize * boxsize * boxsize);
+        double rlow = 0.0;
+        for (int i = 0; i < nbin; i++) {
+            results->npairs[i] = R.np[i];
+            results->rupp[i] = R.rupp[i];
+            results->ravg[i] = options->need_avg_sep ? R.avg[i] : 0.0;
+            results->weightavg[i] = R.need_w ? R.wavg[i] : 0.0;
+            double weight0 = (double)results->npairs[i];
+            if (R.need_w) weight0 *= results->weightavg[i];
+            const double vol = 4.0 / 3.0 * M_PI * (R.rupp[i] * R.rupp[i] * R.rupp[i] - rlow * rlow * rlow);
+            if (vol > 0.0) {
+                double wr = prefac * vol;
+                if (rlow <= 0.) wr += wsq;
+                results->xi[i] = weight0 / wr - 1.0;
+            } else {
+                results->xi[i] = -2.0;
+            }
+            rlow = results->rupp[i];
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int countpairs_wp(const int64_t ND1, void* X1, void* Y1, void* Z1, const double boxsize, const int numthreads,
+                  const char* binfile, const double pimax, results_countpairs_wp* results,
+                  struct config_options* options, struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        const double t0 = now_s();
+        struct extra_options dummy;
+        CallOut R;
+        if (xi_wp_run(MODE_WP, ND1, X1, Y1, Z1, boxsize, numthreads, binfile, pimax, options, extra, &dummy, R))
+            return EXIT_FAILURE;
+        const int nbin = R.nbin;
+        results->nbin = nbin;
+        results->pimax = pimax;
+        results->npairs = (uint64_t*)malloc(sizeof(uint64_t) * nbin);
+        results->wp = (double*)malloc(sizeof(double) * nbin);
+        results->rupp = (double*)malloc(sizeof(double) * nbin);
+        results->rpavg = (double*)malloc(sizeof(double) * nbin);
+        results->weightavg = (double*)calloc(nbin, sizeof(double));
+        if (!results->npairs || !results->wp || !results->rupp || !results->rpavg || !results->weightavg) {
+            free_results_wp(results);
+            return EXIT_FAILURE;
+        }
+        // analytic RR (countpairs_wp_impl.c.src:619-659)
+        double wsum = (double)ND1, wsq = (double)ND1;
+        if (R.need_w) {
+            wsum = R.wsum;
+            wsq = (double)ND1 + R.wsq;
+        }
+        const double prefac = wsum * (wsum - wsum / ND1) / (boxsize * boxsize * boxsize);
+        double rlow = 0.0;
+        const double twice_pimax = 2.0 * pimax;
+        for (int i = 0; i < nbin; i++) {
+            results->npairs[i] = R.np[i];
+            results->rupp[i] = R.rupp[i];
+            results->rpavg[i] = options->need_avg_sep ? R.avg[i] : 0.0;
+            results->weightavg[i] = R.need_w ? R.wavg[i] : 0.0;
+            double weight0 = (double)results->npairs[i];
+            if (R.need_w) weight0 *= results->weightavg[i];
+            const double vol = M_PI * (results->rupp[i] * results->rupp[i] - rlow * rlow) * twice_pimax;
+            if (vol > 0.0) {
+                double wr = prefac * vol;
+                if (rlow <= 0.) wr += wsq;
+                results->wp[i] = (weight0 / wr - 1) * twice_pimax;
+            } else {
+                results->wp[i] = -2.0 * twice_pimax;
+            }
+            rlow = results->rupp[i];
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+// ------------------------------------------------------------ extensions
+int cf_amd_set_engine(int kind)
+{
+    return guarded([&]() -> int {
+        if (kind != CF_AMD_ENGINE_TUNED && kind != CF_AMD_ENGINE_PLAIN) return EXIT_FAILURE;
+        eng().engine_kind = kind;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_get_engine(void) { return guarded([&]() -> int { return eng().engine_kind; }); }
+
+int cf_amd_last_stats(cf_amd_stats* out)
+{
+    if (!out) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        *out = eng().last;
+        return EXIT_SUCCESS;
+    });
+}
+
+const char* cf_amd_version(void) { return "corrfunc_amd 0.1 (reference API 2.5.3, gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,756 @@
+// cf_kernels.hip -- HIP kernels of the MI355X theory pair-counting engine (gfx950).
+//
+// Device pipeline of one countpairs* call:
+//   k_minmax / k_minmax_final     data extent              (reference gridlink_utils.c.src:52-70)
+//   k_bucket_count                cell index + z-slab per particle, bucket histogram
+//                                                          (gridlink_impl.c.src:162-201)
+//   k_scan_*                      exclusive prefix sum of the bucket histogram
+//   k_scatter                     counting-sort scatter into SoA arrays in HBM (FP64 + FP32 local)
+//   k_cell_bounds                 per-cell bounding boxes  (gridlink_impl.c.src:292-316)
+//   k_cellpair_flag / _emit       neighbour-cell work list with periodic wraps and
+//                                 min-separation culling   (gridlink_impl.c.src:439-625)
+//   k_pairs_plain                 FP64 pair kernel, one workgroup per cell pair
+//   k_pairs_tuned                 FP32 filter + FP64 replay, persistent waves, LDS histograms
+//
+// All FP64 arithmetic that decides a pair's bin is written in the reference's
+// operation order; this file is compiled with -ffp-contract=off so the compiler
+// never fuses it, and fma() appears only where the reference's AVX512F kernels
+// use an FMA.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+#include "cf_device.h"
+
+namespace cfamd {
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double wave_min_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ------------------------------------------------------------ extent (a2)
+template <typename T>
+__global__ void k_minmax(int64_t n, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
+                         double* __restrict__ partial)
+{
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v[3] = {(double)X[i], (double)Y[i], (double)Z[i]};
+        for (int d = 0; d < 3; d++) { mn[d] = fmin(mn[d], v[d]); mx[d] = fmax(mx[d], v[d]); }
+    }
+    __shared__ double s[6][256];
+    for (int d = 0; d < 3; d++) { s[d][threadIdx.x] = mn[d]; s[3 + d][threadIdx.x] = mx[d]; }
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int d = 0; d < 3; d++) {
+                s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + st]);
+                s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + st]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 6; d++) partial[blockIdx.x * 6 + d] = s[d][0];
+}
+
+__global__ void k_minmax_final(int nblocks, const double* __restrict__ partial, double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double r[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = 0; b < nblocks; b++)
+        for (int d = 0; d < 3; d++) {
+            r[d] = fmin(r[d], partial[b * 6 + d]);
+            r[3 + d] = fmax(r[3 + d], partial[b * 6 + 3 + d]);
+        }
+    for (int d = 0; d < 6; d++) out[d] = r[d];
+}
+
+// --------------------------------------------------------- gridding (a5)
+struct GridArgs {
+    double xmin, ymin, zmin;
+    double xinv, yinv, zinv;
+    double xbs, ybs, zbs;
+    int nx, ny, nz;
+};
+
+// Cell index exactly as the reference computes it (gridlink_impl.c.src:167-178),
+// followed by the z-slab inside the cell.
+__device__ __forceinline__ int cell_and_slab(const GridArgs& g, double X, double Y, double Z, int* cell, int* oob)
+{
+    int ix = (int)((X - g.xmin) * g.xinv);
+    int iy = (int)((Y - g.ymin) * g.yinv);
+    const double zr = (Z - g.zmin) * g.zinv;
+    int iz = (int)zr;
+    if (ix > g.nx - 1) ix--;
+    if (iy > g.ny - 1) iy--;
+    if (iz > g.nz - 1) iz--;
+    *oob = ix < 0 || ix >= g.nx || iy < 0 || iy >= g.ny || iz < 0 || iz >= g.nz;
+    *cell = (ix * g.ny + iy) * g.nz + iz;
+    int s = (int)((zr - (double)iz) * kSlabs);
+    s = s < 0 ? 0 : (s > kSlabs - 1 ? kSlabs - 1 : s);
+    return s;
+}
+
+template <typename T>
+__global__ void k_bucket_count(int64_t n, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
+                               GridArgs g, int* __restrict__ bucket, int* __restrict__ counts,
+                               unsigned long long* __restrict__ oob_count)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c, oob;
+    const int s = cell_and_slab(g, (double)X[i], (double)Y[i], (double)Z[i], &c, &oob);
+    if (oob) {
+        atomicAdd(oob_count, 1ull);
+        bucket[i] = -1;
+        return;
+    }
+    const int b = c * kSlabs + s;
+    bucket[i] = b;
+    atomicAdd(&counts[b], 1);
+}
+
+// Exclusive scan over int32 counts, three phases, 1024 items per block.
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;
+
+__device__ int block_exclusive_scan(int v, int* total)
+{
+    __shared__ int ws[kScanBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int k = 0; k < kScanBlock / 64; k++) {
+        if (k < w) off += ws[k];
+        tot += ws[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+__global__ void k_scan_blocks(int64_t n, const int* __restrict__ in, int* __restrict__ out, int* __restrict__ block_sums)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock * kScanItems + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems], s = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_exclusive_scan(s, &tot);
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_add(int64_t n, int* __restrict__ out, const int* __restrict__ block_offsets)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock * kScanItems + (int64_t)threadIdx.x * kScanItems;
+    const int add = block_offsets[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) out[base + k] += add;
+}
+
+template <typename T>
+__global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
+                          const T* __restrict__ W, GridArgs g, const int* __restrict__ bucket,
+                          const int* __restrict__ bstart, int* __restrict__ fill,
+                          double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+                          double* __restrict__ w, float4* __restrict__ f4)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = bucket[i];
+    const int pos = bstart[b] + atomicAdd(&fill[b], 1);
+    const double xv = (double)X[i], yv = (double)Y[i], zv = (double)Z[i];
+    x[pos] = xv;
+    y[pos] = yv;
+    z[pos] = zv;
+    if (W) w[pos] = (double)W[i];
+    const int c = b / kSlabs;
+    const int iz = c % g.nz, ixy = c / g.nz, iy = ixy % g.ny, ix = ixy / g.ny;
+    const double ox = g.xmin + ix * g.xbs, oy = g.ymin + iy * g.ybs, oz = g.zmin + iz * g.zbs;
+    f4[pos] = make_float4((float)(xv - ox), (float)(yv - oy), (float)(zv - oz), 0.f);
+}
+
+__global__ void k_cell_bounds(int64_t ncells, const int* __restrict__ bstart, const double* __restrict__ x,
+                              const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ bounds)
+{
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    double b[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
+    for (int k = bstart[c * kSlabs]; k < bstart[(c + 1) * kSlabs]; k++) {
+        b[0] = x[k] < b[0] ? x[k] : b[0];
+        b[1] = x[k] > b[1] ? x[k] : b[1];
+        b[2] = y[k] < b[2] ? y[k] : b[2];
+        b[3] = y[k] > b[3] ? y[k] : b[3];
+        b[4] = z[k] < b[4] ? z[k] : b[4];
+        b[5] = z[k] > b[5] ? z[k] : b[5];
+    }
+    for (int d = 0; d < 6; d++) bounds[c * 6 + d] = b[d];
+}
+
+__global__ void k_weight_sums(int64_t n, const double* __restrict__ w, double* __restrict__ out)
+{
+    double s = 0, s2 = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        s += w[i];
+        s2 += w[i] * w[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], s);
+        atomicAdd(&out[1], s2);
+    }
+}
+
+// ------------------------------------------------------- cell pairs (a6)
+struct NgbArgs {
+    int nx, ny, nz;
+    int rx, ry, rz;
+    int px, py, pz;
+    int autocorr, min_sep_opt, dupcheck;
+    double xwrap, ywrap, zwrap;
+    double max3, max2, max1;
+};
+
+__device__ __forceinline__ int64_t cell_count(const int* bstart, int64_t c)
+{
+    return (int64_t)bstart[(c + 1) * kSlabs] - bstart[c * kSlabs];
+}
+
+// Evaluates neighbour k of primary cell icell in the reference's loop order
+// (iix outer, iiy, iiz inner). Returns 1 and fills rec when the reference would
+// append this cell pair (before its duplicate check).
+__device__ int eval_neighbour(const NgbArgs& a, int64_t icell, int k, const int* __restrict__ bs1,
+                              const int* __restrict__ bs2, const double* __restrict__ bd1,
+                              const double* __restrict__ bd2, CellPairRec* rec)
+{
+    const int ny_n = 2 * a.ry + 1, nz_n = 2 * a.rz + 1;
+    const int iix = k / (ny_n * nz_n) - a.rx;
+    const int iiy = (k / nz_n) % ny_n - a.ry;
+    const int iiz = k % nz_n - a.rz;
+    const int iz = (int)(icell % a.nz);
+    const int ix = (int)(icell / ((int64_t)a.ny * a.nz));
+    const int iy = (int)((icell - iz - (int64_t)ix * a.nz * a.ny) / a.nz);
+    const int iiix = a.px ? (ix + iix + a.nx) % a.nx : ix + iix;
+    if (iiix < 0 || iiix >= a.nx) return 0;
+    const int iiiy = a.py ? (iy + iiy + a.ny) % a.ny : iy + iiy;
+    if (iiiy < 0 || iiiy >= a.ny) return 0;
+    const int iiiz = a.pz ? (iz + iiz + a.nz) % a.nz : iz + iiz;
+    if (iiiz < 0 || iiiz >= a.nz) return 0;
+    const int wx = (ix + iix >= 0 && ix + iix < a.nx) ? 0 : (ix + iix < 0 ? 1 : -1);
+    const int wy = (iy + iiy >= 0 && iy + iiy < a.ny) ? 0 : (iy + iiy < 0 ? 1 : -1);
+    const int wz = (iz + iiz >= 0 && iz + iiz < a.nz) ? 0 : (iz + iiz < 0 ? 1 : -1);
+    const int64_t icell2 = iiiz + (int64_t)a.nz * iiiy + (int64_t)a.nz * a.ny * iiix;
+    if ((a.autocorr == 1 && icell2 > icell) || cell_count(bs2, icell2) == 0) return 0;
+    const double offx = wx * a.xwrap, offy = wy * a.ywrap, offz = wz * a.zwrap;
+    double cx = 0, cy = 0, cz = 0, mdx = 0, mdy = 0, mdz = 0;
+    if (a.min_sep_opt) {
+        const double* f = bd1 + icell * 6;
+        const double* s = bd2 + icell2 * 6;
+        const double xl = f[0] + offx, xh = f[1] + offx;
+        const double yl = f[2] + offy, yh = f[3] + offy;
+        const double zl = f[4] + offz, zh = f[5] + offz;
+        cx = iix < 0 ? xl : (iix > 0 ? xh : 0.0);
+        cy = iiy < 0 ? yl : (iiy > 0 ? yh : 0.0);
+        cz = iiz < 0 ? zl : (iiz > 0 ? zh : 0.0);
+        mdx = iix != 0 ? ((iix < 0 ? xl : xh) - (iix < 0 ? s[1] : s[0])) : 0.0;
+        mdy = iiy != 0 ? ((iiy < 0 ? yl : yh) - (iiy < 0 ? s[3] : s[2])) : 0.0;
+        mdz = iiz != 0 ? ((iiz < 0 ? zl : zh) - (iiz < 0 ? s[5] : s[4])) : 0.0;
+        if (a.max3 > 0) {
+            const double t = mdx * mdx + mdy * mdy + mdz * mdz;
+            if (t >= a.max3 * a.max3) return 0;
+        }
+        if (a.max2 > 0) {
+            const double t = mdx * mdx + mdy * mdy;
+            if (t >= a.max2 * a.max2) return 0;
+        }
+        if (a.max1 > 0 && iiz != 0) {
+            if (mdz * mdz >= a.max1 * a.max1) return 0;
+        }
+    }
+    rec->c1 = (int)icell;
+    rec->c2 = (int)icell2;
+    rec->wx = (signed char)wx;
+    rec->wy = (signed char)wy;
+    rec->wz = (signed char)wz;
+    rec->same = (a.autocorr == 1 && icell2 == icell) ? 1 : 0;
+    rec->pad = 0;
+    rec->mdx = mdx; rec->mdy = mdy; rec->mdz = mdz;
+    rec->cx = cx; rec->cy = cy; rec->cz = cz;
+    return 1;
+}
+
+__device__ int neighbour_added(const NgbArgs& a, int64_t icell, int k, const int* bs1, const int* bs2,
+                               const double* bd1, const double* bd2, CellPairRec* rec)
+{
+    if (cell_count(bs1, icell) == 0) return 0;
+    if (!eval_neighbour(a, icell, k, bs1, bs2, bd1, bd2, rec)) return 0;
+    if (a.dupcheck) {
+        // the reference skips a neighbour already appended for this primary cell with
+        // the same secondary cell and the same wrap (gridlink_utils.h.src:46-72)
+        CellPairRec prev;
+        for (int kk = 0; kk < k; kk++) {
+            if (eval_neighbour(a, icell, kk, bs1, bs2, bd1, bd2, &prev) && prev.c2 == rec->c2 &&
+                prev.wx == rec->wx && prev.wy == rec->wy && prev.wz == rec->wz)
+                return 0;
+        }
+    }
+    return 1;
+}
+
+__global__ void k_cellpair_flag(NgbArgs a, int64_t ncells, int nngb, const int* __restrict__ bs1,
+                                const int* __restrict__ bs2, const double* __restrict__ bd1,
+                                const double* __restrict__ bd2, int* __restrict__ flags)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= ncells * nngb) return;
+    CellPairRec r;
+    flags[t] = neighbour_added(a, t / nngb, (int)(t % nngb), bs1, bs2, bd1, bd2, &r);
+}
+
+__global__ void k_cellpair_emit(NgbArgs a, int64_t ncells, int nngb, const int* __restrict__ bs1,
+                                const int* __restrict__ bs2, const double* __restrict__ bd1,
+                                const double* __restrict__ bd2, const int* __restrict__ pos,
+                                CellPairRec* __restrict__ out, double* __restrict__ tested)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double my = 0;
+    if (t < ncells * nngb) {
+        CellPairRec r;
+        if (neighbour_added(a, t / nngb, (int)(t % nngb), bs1, bs2, bd1, bd2, &r)) {
+            out[pos[t]] = r;
+            const double n1 = (double)cell_count(bs1, r.c1), n2 = (double)cell_count(bs2, r.c2);
+            my = r.same ? n1 * (n1 - 1) * 0.5 : n1 * n2;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) my += __shfl_xor(my, o, 64);
+    if ((threadIdx.x & 63) == 0 && my != 0) atomicAdd(tested, my);
+}
+
+// ------------------------------------------------- exact pair replay (a8-a12)
+__device__ __forceinline__ double r2_3d(int f, double dx, double dy, double dz)
+{
+    switch (f) {
+    case R2_FMA_DZ_DY_DX: return fma(dz, dz, fma(dy, dy, dx * dx));
+    case R2_X_SUM_YZ: return dx * dx + (dy * dy + dz * dz);
+    case R2_FMA_DX_DY_DZ: return fma(dx, dx, fma(dy, dy, dz * dz));
+    default: return (dx * dx + dy * dy) + dz * dz;
+    }
+}
+
+__device__ __forceinline__ double rp2_f(int f, double dx, double dy)
+{
+    return f == RP_FMA ? fma(dy, dy, dx * dx) : dx * dx + dy * dy;
+}
+
+// first k from the top with r2 >= rupp_sqr[k-1] (countpairs_kernels.c.src:982-993)
+__device__ __forceinline__ int find_kbin(const double* __restrict__ e, int nbin, int sorted, double r2)
+{
+    if (sorted) {
+        int lo = 0, hi = nbin - 1;  // count of e[0..nbin-2] <= r2
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (e[mid] <= r2) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    }
+    for (int k = nbin - 1; k >= 1; k--)
+        if (r2 >= e[k - 1]) return k;
+    return 0;
+}
+
+struct ITerms {
+    double xpos, ypos, zpos;
+    double max_dz, target;
+    int skip;
+};
+
+// The reference kernels' per-particle prologue (countpairs_kernels.c.src:900-950 and
+// the rp_pi / s_mu / wp / xi counterparts): position + wrap, minimum separation of
+// this particle to the secondary cell, z window.
+__device__ __forceinline__ void i_terms(const PairParams& P, int mode, const CellPairRec& cp, double xi, double yi,
+                                        double zi, ITerms* t)
+{
+    t->xpos = xi + cp.wx * P.xwrap;
+    t->ypos = yi + cp.wy * P.ywrap;
+    t->zpos = zi + cp.wz * P.zwrap;
+    if (cp.same) {
+        t->skip = 0;
+        t->max_dz = (mode == MODE_RPPI || mode == MODE_WP) ? P.pimax : P.max_all_dz_same;
+        t->target = t->zpos - t->max_dz;
+        return;
+    }
+    const double mdx = cp.mdx > 0 ? cp.mdx + fabs(t->xpos - cp.cx) : cp.mdx;
+    const double mdy = cp.mdy > 0 ? cp.mdy + fabs(t->ypos - cp.cy) : cp.mdy;
+    const double mdz = cp.mdz > 0 ? cp.mdz + fabs(t->zpos - cp.cz) : cp.mdz;
+    if (mode == MODE_RPPI || mode == MODE_WP) {
+        t->skip = (mdx * mdx + mdy * mdy) >= P.sqr_rmax || mdz >= P.pimax;
+        t->max_dz = P.pimax;
+        t->target = t->zpos - P.pimax;
+    } else if (mode == MODE_SMU) {
+        t->skip = ((mdx * mdx + mdy * mdy) + mdz * mdz) >= P.sqr_rmax || mdz >= P.pimax;
+        const double sq = (P.sqr_rmax - mdx * mdx) - mdy * mdy;
+        t->max_dz = sq < P.pimax * P.pimax ? sqrt(sq) : P.pimax;
+        t->target = t->zpos - t->max_dz;
+    } else {
+        t->skip = ((mdx * mdx + mdy * mdy) + mdz * mdz) >= P.sqr_rmax;
+        t->max_dz = sqrt((P.sqr_rmax - mdx * mdx) - mdy * mdy);
+        t->target = t->zpos - t->max_dz;
+    }
+}
+
+// Decides one (i, j) pair exactly as the reference does. Returns the histogram
+// slot or -1; *sep receives sqrt(r^2) (or sqrt(s^2)).
+// `window` = 0 skips the z-window / per-particle tests, which the caller may do
+// only when it has shown the pair lies outside the edge windows (see k_pairs_tuned).
+__device__ __forceinline__ int replay_pair(const PairParams& P, int mode, const ITerms& t, int same, int window,
+                                           double xj, double yj, double zj, const double* __restrict__ e,
+                                           double* sep)
+{
+    const double dx = xj - t.xpos, dy = yj - t.ypos, dz = zj - t.zpos;
+    if (mode == MODE_RPPI || mode == MODE_WP) {
+        if (window && ((!same && (t.skip || zj <= t.target)))) return -1;
+        if (dz <= -P.pimax || dz >= P.pimax) return -1;
+        const double adz = fabs(dz);
+        const double r2 = rp2_f(P.r2form, dx, dy);
+        if (r2 >= P.sqr_rmax || r2 < P.sqr_rmin) return -1;
+        const int k = find_kbin(e, P.nbin, P.sorted_edges, r2);
+        if (k == 0) return -1;
+        *sep = sqrt(r2);
+        if (mode == MODE_WP) return k;
+        if (P.float_index) return (int)((double)k * (double)(P.n2 + 1) + adz * P.inv_d2);
+        int pb = (int)(adz * P.inv_d2);
+        pb = pb > P.n2 ? P.n2 : pb;
+        return k * (P.n2 + 1) + pb;
+    }
+    if (window) {
+        if (t.skip) return -1;
+        if (zj <= t.target) return -1;
+        if (dz >= t.max_dz) return -1;
+    }
+    if (mode == MODE_SMU) {
+        const double sqr_dz = dz * dz;
+        const double s2 = P.r2form == R2_FMA_DX_DY_DZ ? fma(dx, dx, fma(dy, dy, sqr_dz)) : (dx * dx + dy * dy) + sqr_dz;
+        if (s2 >= P.sqr_rmax || s2 < P.sqr_rmin) return -1;
+        if (sqr_dz >= s2 * P.sqr_mu_max) return -1;
+        const double mu = sqrt(sqr_dz / s2);
+        const int k = find_kbin(e, P.nbin, P.sorted_edges, s2);
+        if (k == 0) return -1;
+        *sep = sqrt(s2);
+        if (P.float_index) return (int)((double)k * (double)(P.n2 + 1) + mu * P.inv_d2);
+        int mb = (int)(mu * P.inv_d2);
+        mb = mb > P.n2 ? P.n2 : mb;
+        return k * (P.n2 + 1) + mb;
+    }
+    const double r2 = r2_3d(P.r2form, dx, dy, dz);
+    if (r2 >= P.sqr_rmax || r2 < P.sqr_rmin) return -1;
+    const int k = find_kbin(e, P.nbin, P.sorted_edges, r2);
+    if (k == 0) return -1;
+    *sep = sqrt(r2);
+    return k;
+}
+
+// Histogram of one workgroup in LDS: counts (u64), sum of separations, sum of weights.
+struct LdsHist {
+    unsigned long long* np;
+    double* avg;
+    double* w;
+};
+
+__device__ __forceinline__ void hist_add(const PairParams& P, LdsHist& h, int slot, double sep, double wprod)
+{
+    atomicAdd(&h.np[slot], 1ull);
+    if (P.need_avg) atomicAdd(&h.avg[slot], sep);
+    if (P.need_w) atomicAdd(&h.w[slot], wprod);
+}
+
+__device__ void hist_flush(const PairParams& P, LdsHist& h, unsigned long long* g_np, double* g_avg, double* g_w)
+{
+    for (int s = threadIdx.x; s < P.nslots; s += blockDim.x) {
+        const unsigned long long c = h.np[s];
+        if (c) {
+            atomicAdd(&g_np[s], c);
+            if (P.need_avg) atomicAdd(&g_avg[s], h.avg[s]);
+            if (P.need_w) atomicAdd(&g_w[s], h.w[s]);
+        }
+    }
+}
+
+__device__ void hist_init(const PairParams& P, LdsHist& h)
+{
+    for (int s = threadIdx.x; s < P.nslots; s += blockDim.x) {
+        h.np[s] = 0;
+        h.avg[s] = 0;
+        h.w[s] = 0;
+    }
+}
+
+// -------------------------------------------------- plain FP64 pair kernel
+// One workgroup per cell pair; every (i, j) of the pair is decided in FP64 with
+// the full reference predicate. This is the "plain GPU form".
+__global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1, DevLattice L2,
+                                                     const CellPairRec* __restrict__ cps,
+                                                     const double* __restrict__ rupp_sqr,
+                                                     unsigned long long* __restrict__ g_np,
+                                                     double* __restrict__ g_avg, double* __restrict__ g_w)
+{
+    extern __shared__ unsigned long long smem_u64[];
+    LdsHist h;
+    h.np = smem_u64;
+    h.avg = reinterpret_cast<double*>(smem_u64 + P.nslots);
+    h.w = h.avg + P.nslots;
+    double* e = h.w + P.nslots;
+    for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
+    hist_init(P, h);
+    __syncthreads();
+    const CellPairRec cp = cps[blockIdx.x];
+    const int mode = P.mode;
+    const int i0 = L1.bstart[(int64_t)cp.c1 * kSlabs], i1 = L1.bstart[(int64_t)(cp.c1 + 1) * kSlabs];
+    const int j0 = L2.bstart[(int64_t)cp.c2 * kSlabs], j1 = L2.bstart[(int64_t)(cp.c2 + 1) * kSlabs];
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        ITerms t;
+        i_terms(P, mode, cp, L1.x[i], L1.y[i], L1.z[i], &t);
+        if (t.skip) continue;
+        const double wi = P.need_w ? L1.w[i] : 0.0;
+        const int jstart = cp.same ? i + 1 : j0;
+        for (int j = jstart; j < j1; j++) {
+            double sep = 0;
+            const int slot = replay_pair(P, mode, t, cp.same, 1, L2.x[j], L2.y[j], L2.z[j], e, &sep);
+            if (slot >= 0) hist_add(P, h, slot, sep, P.need_w ? wi * L2.w[j] : 0.0);
+        }
+    }
+    __syncthreads();
+    hist_flush(P, h, g_np, g_avg, g_w);
+}
+
+// ------------------------------------------------- tuned pair kernel
+// Work item: (cell pair, chunk of <= 64 consecutive primary particles).
+// Each wave pulls work items from a device counter until the list is exhausted.
+// Lane l holds primary particle i0+l; the secondary particles of the cell pair
+// (restricted to the z-slabs the chunk can reach) are streamed through the
+// scalar unit, since their address is uniform across the wave. An FP32 test on
+// cell-local coordinates, with a proven error margin, rejects pairs that cannot
+// be in range; the survivors are compacted into a per-wave LDS queue and decided
+// 64 at a time by the exact FP64 replay, so counts and sums equal the FP64 path.
+struct WorkItem {
+    int cp;
+    int i0, i1;
+    int pad;
+};
+
+constexpr int kQueue = 128;
+
+// Secondary particles are read through the constant address space so that the
+// wave-uniform loads become scalar (SMEM) loads into SGPRs.
+struct __attribute__((aligned(16))) F4 {
+    float x, y, z, w;
+};
+typedef const __attribute__((address_space(4))) F4* uniform_f4_ptr;
+
+__device__ __forceinline__ void drain_queue(const PairParams& P, int mode, const CellPairRec& cp,
+                                            const DevLattice& L1, const DevLattice& L2, int i0,
+                                            const int* qj, const unsigned char* ql, int head, int ndrain,
+                                            const double* e, LdsHist& h)
+{
+    const int lane = lane_id();
+    if (lane < ndrain) {
+        const int slotq = (head + lane) & (kQueue - 1);
+        const int j = qj[slotq];
+        const int i = i0 + ql[slotq];
+        ITerms t;
+        t.xpos = L1.x[i] + cp.wx * P.xwrap;
+        t.ypos = L1.y[i] + cp.wy * P.ywrap;
+        t.zpos = L1.z[i] + cp.wz * P.zwrap;
+        t.skip = 0;
+        t.max_dz = 0;
+        t.target = 0;
+        const double xj = L2.x[j], yj = L2.y[j], zj = L2.z[j];
+        // pairs near an edge of the z window or of the largest bin replay the
+        // reference's per-particle pruning tests; all others provably pass them
+        const double dx = xj - t.xpos, dy = yj - t.ypos, dz = zj - t.zpos;
+        double q2;
+        if (mode == MODE_RPPI || mode == MODE_WP) q2 = dx * dx + dy * dy;
+        else q2 = (dx * dx + dy * dy) + dz * dz;
+        const int near = q2 >= P.sqr_rmax - P.near_r2 ||
+                         ((mode != MODE_DD && mode != MODE_XI) && fabs(dz) >= P.pimax - P.near_dz);
+        if (near) i_terms(P, mode, cp, L1.x[i], L1.y[i], L1.z[i], &t);
+        double sep = 0;
+        const int slot = replay_pair(P, mode, t, cp.same, near, xj, yj, zj, e, &sep);
+        if (slot >= 0) hist_add(P, h, slot, sep, P.need_w ? L1.w[i] * L2.w[j] : 0.0);
+    }
+}
+
+template <int MODEK>  // 0: DD/xi (3-D r), 1: rp_pi/wp, 2: s_mu
+__global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1, DevLattice L2,
+                                                     const CellPairRec* __restrict__ cps,
+                                                     const WorkItem* __restrict__ items, int nitems,
+                                                     int* __restrict__ counter,
+                                                     const double* __restrict__ rupp_sqr,
+                                                     unsigned long long* __restrict__ g_np,
+                                                     double* __restrict__ g_avg, double* __restrict__ g_w)
+{
+    extern __shared__ unsigned long long smem_u64[];
+    LdsHist h;
+    h.np = smem_u64;
+    h.avg = reinterpret_cast<double*>(smem_u64 + P.nslots);
+    h.w = h.avg + P.nslots;
+    double* e = h.w + P.nslots;
+    int* qbase = reinterpret_cast<int*>(e + P.nbin);
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    int* qj = qbase + wave * kQueue;
+    unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + (blockDim.x >> 6) * kQueue) + wave * kQueue;
+    for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
+    hist_init(P, h);
+    __syncthreads();
+    const int mode = P.mode;
+    const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
+    uniform_f4_ptr f4j = (uniform_f4_ptr)L2.f4;
+
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(counter, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= nitems) break;
+        const WorkItem it = items[item];
+        const CellPairRec cp = cps[__builtin_amdgcn_readfirstlane(it.cp)];
+        const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
+        const int i = i0 + lane;
+        const int ivalid = i < i1;
+        const int c2 = cp.c2;
+        const int iz2 = c2 % L2.nz, ixy2 = c2 / L2.nz, iy2 = ixy2 % L2.ny, ix2 = ixy2 / L2.ny;
+        const double ox = L2.xmin + ix2 * L2.xbs, oy = L2.ymin + iy2 * L2.ybs, oz = L2.zmin + iz2 * L2.zbs;
+        double zpos = 0;
+        float xr = 0.f, yr = 0.f, zr_i = 0.f;
+        if (ivalid) {
+            const double xpos = L1.x[i] + cp.wx * P.xwrap;
+            const double ypos = L1.y[i] + cp.wy * P.ywrap;
+            zpos = L1.z[i] + cp.wz * P.zwrap;
+            xr = (float)(xpos - ox);
+            yr = (float)(ypos - oy);
+            zr_i = (float)(zpos - oz);
+        }
+        // secondary z-slabs this chunk can reach
+        const double zlo = wave_min_d(ivalid ? zpos : DBL_MAX) - P.zreach - P.slab_margin;
+        const double zhi = wave_max_d(ivalid ? zpos : -DBL_MAX) + P.zreach + P.slab_margin;
+        int jlo, jhi;
+        {
+            const int64_t cb = (int64_t)c2 * kSlabs;
+            double flo = floor(((zlo - L2.zmin) * L2.zinv - (double)iz2) * kSlabs);
+            double fhi = floor(((zhi - L2.zmin) * L2.zinv - (double)iz2) * kSlabs);
+            if (L2.zinv == 0.0) { flo = 0.0; fhi = kSlabs - 1; }
+            if (fhi < 0.0 || flo > (double)(kSlabs - 1)) {
+                jlo = jhi = 0;
+            } else {
+                const int slo = flo < 0.0 ? 0 : (int)flo;
+                const int shi = fhi > (double)(kSlabs - 1) ? kSlabs - 1 : (int)fhi;
+                jlo = L2.bstart[cb + slo];
+                jhi = L2.bstart[cb + shi + 1];
+            }
+        }
+        if (cp.same && jlo < i0 + 1) jlo = i0 + 1;
+        jlo = __builtin_amdgcn_readfirstlane(jlo);
+        jhi = __builtin_amdgcn_readfirstlane(jhi);
+
+        int head = 0, tail = 0, count = 0;
+        for (int j = jlo; j < jhi; j++) {
+            const float pjx = f4j[j].x, pjy = f4j[j].y, pjz = f4j[j].z;
+            const float dx = pjx - xr, dy = pjy - yr, dz = pjz - zr_i;
+            float q;
+            int pass;
+            if (MODEK == 1) {
+                q = fmaf(dy, dy, dx * dx);
+                pass = ivalid && q < hi2 && q >= lo2 && fabsf(dz) < zr;
+            } else {
+                q = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                pass = ivalid && q < hi2 && q >= lo2;
+            }
+            if (cp.same) pass = pass && (j > i);
+            const unsigned long long m = __ballot(pass);
+            if (m) {
+                if (pass) {
+                    const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+                    const int pos = (tail + before) & (kQueue - 1);
+                    qj[pos] = j;
+                    ql[pos] = (unsigned char)lane;
+                }
+                const int added = __popcll(m);
+                tail += added;
+                count += added;
+                if (count >= 64) {
+                    __builtin_amdgcn_wave_barrier();
+                    drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, 64, e, h);
+                    __builtin_amdgcn_wave_barrier();
+                    head += 64;
+                    count -= 64;
+                }
+            }
+        }
+        if (count > 0) {
+            __builtin_amdgcn_wave_barrier();
+            drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, count, e, h);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    __syncthreads();
+    hist_flush(P, h, g_np, g_avg, g_w);
+}
+
+template __global__ void k_pairs_tuned<0>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
+                                          int*, const double*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<1>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
+                                          int*, const double*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<2>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
+                                          int*, const double*, unsigned long long*, double*, double*);
+
+// Work-item construction: chunks of 64 primary particles per cell pair.
+__global__ void k_count_chunks(int ncp, const CellPairRec* __restrict__ cps, const int* __restrict__ bs1,
+                               int* __restrict__ nchunks)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const int n = (int)cell_count(bs1, cps[t].c1);
+    nchunks[t] = (n + 63) / 64;
+}
+
+__global__ void k_emit_chunks(int ncp, const CellPairRec* __restrict__ cps, const int* __restrict__ bs1,
+                              const int* __restrict__ off, WorkItem* __restrict__ items)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const int c1 = cps[t].c1;
+    const int s = bs1[(int64_t)c1 * kSlabs], e = bs1[(int64_t)(c1 + 1) * kSlabs];
+    int o = off[t];
+    for (int i = s; i < e; i += 64) {
+        WorkItem w;
+        w.cp = t;
+        w.i0 = i;
+        w.i1 = i + 64 < e ? i + 64 : e;
+        w.pad = 0;
+        items[o++] = w;
+    }
+}
+
+}  // namespace cfamd

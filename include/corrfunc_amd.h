@@ -1,0 +1,55 @@
+/*
+ * corrfunc_amd.h -- extensions of the MI355X engine beyond the reference C API.
+ *
+ * None of these has a counterpart in the reference (Corrfunc v2.5.3); they exist
+ * for callers that benchmark or steer the GPU engine. The reference entry points
+ * in corrfunc_theory.h work without them.
+ *
+ * Input residency: every entry point of corrfunc_theory.h accepts position and
+ * weight pointers in host memory (as the reference does; they are copied to the
+ * GPU) or in GPU memory (detected with hipPointerGetAttributes and used in place).
+ */
+#ifndef CORRFUNC_AMD_H
+#define CORRFUNC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Pair-kernel variants. TUNED (default): FP32 filter on cell-local coordinates,
+ * FP64 replay of the surviving pairs, persistent waves. PLAIN: every pair of
+ * every cell pair decided in FP64, one workgroup per cell pair. Both give the
+ * same counts. The environment variable CORRFUNC_AMD_ENGINE=plain selects PLAIN
+ * at load time. */
+#define CF_AMD_ENGINE_TUNED 0
+#define CF_AMD_ENGINE_PLAIN 1
+
+typedef struct {
+    int nmesh[3];            /* grid used by the last call */
+    int refine[3];           /* bin refine factors used by the last call */
+    int64_t num_cell_pairs;  /* neighbour-cell pairs in the work list */
+    int64_t num_work_items;  /* work items of the tuned kernel (0 for PLAIN) */
+    double tested_pairs;     /* sum over cell pairs of N1*N2 (N(N-1)/2 for a cell with itself) */
+    double kernel_ms;        /* pair-kernel time, HIP events on the engine's stream */
+    double grid_ms;          /* host wall time from entry to the end of gridding */
+    double total_ms;         /* host wall time of the engine part of the call */
+    int engine;              /* CF_AMD_ENGINE_* used */
+    int pad;
+} cf_amd_stats;
+
+/* Selects the pair kernel for subsequent calls; returns EXIT_SUCCESS/EXIT_FAILURE. */
+int cf_amd_set_engine(int kind);
+/* Returns the pair kernel currently selected. */
+int cf_amd_get_engine(void);
+/* Copies the statistics of the last successful call. */
+int cf_amd_last_stats(cf_amd_stats *out);
+/* Library identification string. */
+const char *cf_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CORRFUNC_AMD_H */

@@ -12,12 +12,12 @@
 # Nothing is stubbed: every header and source compiled is the reference's own.
 # The temporary directory is deleted afterwards; only the .so lands in oracle/_ref/.
 #
-# Usage: oracle/build_ref.sh [march]      (default march: x86-64-v3)
+# Usage: oracle/build_ref.sh [march]      (default march: x86-64-v4)
 set -euo pipefail
 REF=${CORRFUNC_REFERENCE:-/root/reference}
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 OUT="$HERE/_ref"
-MARCH=${1:-x86-64-v3}
+MARCH=${1:-x86-64-v4}
 if [ ! -d "$REF/theory" ]; then
     echo "build_ref.sh: reference sources not found at $REF; skipping" >&2
     exit 0

@@ -1,0 +1,84 @@
+"""GPU engine vs the FP64 oracle on identical seeded inputs, through the C ABI.
+
+npairs must be bit-exact; ravg / savg / weightavg / xi / wp within 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+from corrfunc_amd import _abi, _lib
+from helpers import MODES, assert_same, clustered, mode_kwargs, run, uniform, weights
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = (_lib.ENGINE_TUNED, _lib.ENGINE_PLAIN)
+
+
+def _opts(**kw):
+    return _abi.make_options(**kw)
+
+
+@pytest.mark.parametrize("eng", ENGINES, ids=["tuned", "plain"])
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("isa", ["fallback", "avx512f"])
+def test_modes_uniform(engine, oracle, eng, mode, dtype, isa):
+    _lib.set_engine(eng)
+    L = 200.0
+    X = uniform(20000, L, 11, dtype)
+    W = weights(20000, 12, dtype)
+    bins = np.logspace(np.log10(0.5), np.log10(25.0), 15)
+    for autocorr in ((1, 0) if mode not in ("countpairs_xi", "countpairs_wp") else (1,)):
+        X2 = uniform(15000, L, 13, dtype)
+        W2 = weights(15000, 14, dtype)
+        kw = mode_kwargs(mode, X, W, X2, W2, autocorr=autocorr, bins=bins, L=L, pimax=20.0, mu_max=0.8, nmu_bins=8)
+        o = dict(periodic=True, boxsize=L, need_avg_sep=True, isa=isa)
+        got = run(engine, mode, options=_opts(**o), **kw)
+        want = run(oracle, mode, "oracle_", options=_opts(**o), **kw)
+        assert_same(mode, got, want, what="autocorr=%d" % autocorr)
+
+
+@pytest.mark.parametrize("eng", ENGINES, ids=["tuned", "plain"])
+@pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_s_mu"])
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("refine", [(2, 2, 1), (1, 1, 1), (3, 3, 2)])
+def test_options_clustered(engine, oracle, eng, mode, periodic, refine):
+    _lib.set_engine(eng)
+    L = 300.0
+    X = clustered(30000, L, 21, nparent=40, sigma=3.0)
+    W = weights(30000, 22, np.float32)
+    bins = np.concatenate([[0.0], np.logspace(-1, np.log10(30.0), 12)])
+    kw = mode_kwargs(mode, X, W, autocorr=1, bins=bins, L=L, pimax=35.0, mu_max=1.0, nmu_bins=12)
+    o = dict(periodic=periodic, boxsize=L if periodic else None, need_avg_sep=True, refine=refine, isa="fallback")
+    got = run(engine, mode, options=_opts(**o), **kw)
+    want = run(oracle, mode, "oracle_", options=_opts(**o), **kw)
+    assert_same(mode, got, want)
+
+
+@pytest.mark.parametrize("eng", ENGINES, ids=["tuned", "plain"])
+def test_device_resident_inputs(engine, oracle, eng):
+    """Positions already in HBM (torch tensors) are used in place."""
+    torch = pytest.importorskip("torch")
+    _lib.set_engine(eng)
+    L = 150.0
+    X = uniform(10000, L, 31, np.float64)
+    bins = np.logspace(-0.5, np.log10(20.0), 10)
+    dev = [torch.from_numpy(a).cuda() for a in X]
+    torch.cuda.synchronize()
+    o = _opts(periodic=True, boxsize=L, need_avg_sep=True, isa="fallback")
+    import ctypes as C
+    res = _abi.results_countpairs()
+    path, tmp = _abi.write_bin_file(bins)
+    try:
+        st = engine.countpairs(len(X[0]), C.c_void_p(dev[0].data_ptr()), C.c_void_p(dev[1].data_ptr()),
+                               C.c_void_p(dev[2].data_ptr()), 0, None, None, None, 1, 1, path.encode(),
+                               C.byref(res), C.byref(o), None)
+    finally:
+        import os
+        os.remove(path)
+    assert st == 0
+    got = _abi._collect("countpairs", res)
+    engine.free_results(C.byref(res))
+    want = run(oracle, "countpairs", "oracle_", options=_opts(periodic=True, boxsize=L, need_avg_sep=True,
+                                                               isa="fallback"),
+               autocorr=1, nthreads=2, binfile=bins, X1=X[0], Y1=X[1], Z1=X[2])
+    assert_same("countpairs", got, want)
