@@ -9,6 +9,7 @@ reference's CPython extension (theory/python_bindings/_countpairs.c:1136-1441).
 """
 import ctypes as C
 import os
+import re
 import tempfile
 
 import numpy as np
@@ -224,6 +225,19 @@ def _prep_weights(w, n, dt):
     return np.ascontiguousarray(w, dtype=dt)
 
 
+def highest_host_isa():
+    """The x86 ISA the reference's 'fastest' resolves to on this host
+    (reference utils/cpu_features.c:74 get_max_usable_isa)."""
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return ISA["fallback"]
+    for name, key in (("avx512f", "avx512f"), ("avx2", "avx2"), ("avx", "avx"), ("sse4_2", "sse42")):
+        if re.search(r"\b%s\b" % name, flags):
+            return ISA[key]
+    return ISA["fallback"]
+
+
 def make_options(periodic=True, boxsize=None, verbose=False, need_avg_sep=False, refine=(2, 2, 1),
                  max_cells_per_dim=100, copy_particles=True, enable_min_sep_opt=True,
                  c_api_timer=False, isa="fastest", float_type=8, c_cell_timer=False,
@@ -248,6 +262,9 @@ def make_options(periodic=True, boxsize=None, verbose=False, need_avg_sep=False,
         o.instruction_set = ISA[key]
     else:
         o.instruction_set = int(isa)
+    if o.instruction_set == -1:
+        # as the reference's CPython binding does before calling C
+        o.instruction_set = highest_host_isa()
     if boxsize is not None:
         b = np.atleast_1d(np.asarray(boxsize, dtype=np.float64))
         if len(b) == 1:

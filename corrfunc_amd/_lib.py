@@ -40,6 +40,12 @@ def lib():
         L.cf_amd_last_stats.restype = C.c_int
         L.cf_amd_version.argtypes = []
         L.cf_amd_version.restype = C.c_char_p
+        L.cf_amd_device_alloc.argtypes = [C.c_size_t]
+        L.cf_amd_device_alloc.restype = C.c_void_p
+        L.cf_amd_device_free.argtypes = [C.c_void_p]
+        L.cf_amd_device_free.restype = C.c_int
+        L.cf_amd_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.cf_amd_memcpy.restype = C.c_int
         _lib = L
     return _lib
 
@@ -59,3 +65,38 @@ def last_stats():
     return dict(nmesh=tuple(s.nmesh), refine=tuple(s.refine), num_cell_pairs=s.num_cell_pairs,
                 num_work_items=s.num_work_items, tested_pairs=s.tested_pairs, kernel_ms=s.kernel_ms,
                 grid_ms=s.grid_ms, total_ms=s.total_ms, engine=s.engine)
+
+
+class DeviceArray:
+    """A numpy array copied into HBM through the engine's runtime; `.ptr` is the
+    device address to hand to the C ABI in place of the host pointer."""
+
+    def __init__(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        self.nbytes = a.nbytes
+        self.dtype = a.dtype
+        self.shape = a.shape
+        self.ptr = lib().cf_amd_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError("cf_amd_device_alloc(%d) failed" % self.nbytes)
+        if lib().cf_amd_memcpy(self.ptr, a.ctypes.data, self.nbytes, 0) != 0:
+            raise RuntimeError("copy to device failed")
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty(self.shape, dtype=self.dtype)
+        if lib().cf_amd_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1) != 0:
+            raise RuntimeError("copy from device failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cf_amd_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1130,4 +1130,36 @@ int cf_amd_last_stats(cf_amd_stats* out)
 
 const char* cf_amd_version(void) { return "corrfunc_amd 0.1 (reference API 2.5.3, gfx950)"; }
 
+void* cf_amd_device_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (guarded([&]() -> int {
+            eng();
+            CF_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+            return EXIT_SUCCESS;
+        }) != EXIT_SUCCESS)
+        return nullptr;
+    return p;
+}
+
+int cf_amd_device_free(void* p)
+{
+    return guarded([&]() -> int {
+        CF_CHECK(hipFree(p));
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_memcpy(void* dst, const void* src, size_t bytes, int kind)
+{
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        const hipMemcpyKind k = kind == CF_AMD_H2D ? hipMemcpyHostToDevice
+                              : kind == CF_AMD_D2H ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        CF_CHECK(hipMemcpyAsync(dst, src, bytes, k, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        return EXIT_SUCCESS;
+    });
+}
+
 }  // extern "C"

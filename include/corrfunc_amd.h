@@ -12,6 +12,7 @@
 #ifndef CORRFUNC_AMD_H
 #define CORRFUNC_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -47,6 +48,15 @@ int cf_amd_get_engine(void);
 int cf_amd_last_stats(cf_amd_stats *out);
 /* Library identification string. */
 const char *cf_amd_version(void);
+
+/* Device memory on the engine's GPU, from the engine's own HIP runtime, for
+ * callers that keep catalogs resident in HBM across calls. */
+void *cf_amd_device_alloc(size_t bytes);            /* NULL on failure */
+int cf_amd_device_free(void *p);
+#define CF_AMD_H2D 0
+#define CF_AMD_D2H 1
+#define CF_AMD_D2D 2
+int cf_amd_memcpy(void *dst, const void *src, size_t bytes, int kind);  /* synchronous */
 
 #ifdef __cplusplus
 }

@@ -55,30 +55,35 @@ def test_options_clustered(engine, oracle, eng, mode, periodic, refine):
 
 
 @pytest.mark.parametrize("eng", ENGINES, ids=["tuned", "plain"])
-def test_device_resident_inputs(engine, oracle, eng):
-    """Positions already in HBM (torch tensors) are used in place."""
-    torch = pytest.importorskip("torch")
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_device_resident_inputs(engine, oracle, eng, dtype):
+    """Positions and weights already in HBM are used in place (same result as host inputs)."""
+    import ctypes as C
+    import os
     _lib.set_engine(eng)
     L = 150.0
-    X = uniform(10000, L, 31, np.float64)
+    X = uniform(10000, L, 31, dtype)
+    W = weights(10000, 32, dtype)
     bins = np.logspace(-0.5, np.log10(20.0), 10)
-    dev = [torch.from_numpy(a).cuda() for a in X]
-    torch.cuda.synchronize()
-    o = _opts(periodic=True, boxsize=L, need_avg_sep=True, isa="fallback")
-    import ctypes as C
+    dev = [_lib.DeviceArray(a) for a in X + [W]]
+    o = _opts(periodic=True, boxsize=L, need_avg_sep=True, isa="fallback", float_type=np.dtype(dtype).itemsize)
+    e = _abi.get_extra_options(_abi.WEIGHT_PAIR_PRODUCT)
+    e.weights0.weights[0] = dev[3].ptr
     res = _abi.results_countpairs()
     path, tmp = _abi.write_bin_file(bins)
     try:
-        st = engine.countpairs(len(X[0]), C.c_void_p(dev[0].data_ptr()), C.c_void_p(dev[1].data_ptr()),
-                               C.c_void_p(dev[2].data_ptr()), 0, None, None, None, 1, 1, path.encode(),
-                               C.byref(res), C.byref(o), None)
+        st = engine.countpairs(len(X[0]), dev[0].ptr, dev[1].ptr, dev[2].ptr, 0, None, None, None, 1, 1,
+                               path.encode(), C.byref(res), C.byref(o), C.byref(e))
     finally:
-        import os
         os.remove(path)
     assert st == 0
     got = _abi._collect("countpairs", res)
     engine.free_results(C.byref(res))
+    for a, d in zip(X + [W], dev):
+        assert np.array_equal(a, d.numpy())  # inputs untouched
+        d.free()
     want = run(oracle, "countpairs", "oracle_", options=_opts(periodic=True, boxsize=L, need_avg_sep=True,
                                                                isa="fallback"),
-               autocorr=1, nthreads=2, binfile=bins, X1=X[0], Y1=X[1], Z1=X[2])
+               autocorr=1, nthreads=2, binfile=bins, X1=X[0], Y1=X[1], Z1=X[2], weights1=W,
+               weight_type="pair_product")
     assert_same("countpairs", got, want)
