@@ -41,6 +41,7 @@ struct DevLattice {
     double xbs, ybs, zbs;      // cell sizes
     double xinv, yinv, zinv;   // 1/cell size (0 if the size is 0)
     double *x, *y, *z, *w;     // FP64 positions / weight, cell order
+    double4 *d4;               // the same as one 32-byte record (x, y, z, w) per particle
     float4 *f4;                // cell-local FP32 (x - ox, y - oy, z - oz, 0), ox = xmin + ix*xbs
     int *bstart;               // [ncells*kSlabs + 1] bucket starts
     double *bounds;            // [ncells*6] x lo,hi, y lo,hi, z lo,hi
@@ -67,6 +68,19 @@ struct PairParams {
     float hi2_f, lo2_f, zreach_f, pad_f;
     double zreach;           // z reach used for the slab window
     double slab_margin;
+    // bin lookup: slot count of edges below the 1/16-octave bucket of (float)r^2
+    int lut_base;            // bucket index of lut[0]
+    int lut_n;               // entries
+    int dense_mode;          // 0: by work-item flag, 1: always dense, 2: never dense
+    int pad_i;
 };
+
+// Work item of the tuned kernel: (cell pair, <= 64 consecutive primary particles).
+struct WorkItem {
+    int cp;
+    int i0, i1;
+    int flags;  // bit 0: every pair of the cell pair lies inside the separation range
+};
+constexpr int kItemDense = 1;
 
 }  // namespace cfamd

@@ -265,6 +265,7 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
     L.y = (double*)E.arena.get(tag + "_y", sizeof(double) * n);
     L.z = (double*)E.arena.get(tag + "_z", sizeof(double) * n);
     L.w = W ? (double*)E.arena.get(tag + "_w", sizeof(double) * n) : nullptr;
+    L.d4 = (double4*)E.arena.get(tag + "_d4", sizeof(double4) * n);
     L.f4 = (float4*)E.arena.get(tag + "_f4", sizeof(float4) * n);
     L.bounds = (double*)E.arena.get(tag + "_bounds", sizeof(double) * 6 * L.ncells);
     CF_CHECK(hipMemsetAsync(counts, 0, sizeof(int) * (nb + 1), E.stream));
@@ -293,11 +294,11 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
         if (esize == 4)
             k_scatter<float><<<blocks, 256, 0, E.stream>>>(n, (const float*)X, (const float*)Y, (const float*)Z,
                                                             (const float*)W, ga, bucket, L.bstart, fill, L.x, L.y,
-                                                            L.z, L.w, L.f4);
+                                                            L.z, L.w, L.d4, L.f4);
         else
             k_scatter<double><<<blocks, 256, 0, E.stream>>>(n, (const double*)X, (const double*)Y, (const double*)Z,
                                                              (const double*)W, ga, bucket, L.bstart, fill, L.x,
-                                                             L.y, L.z, L.w, L.f4);
+                                                             L.y, L.z, L.w, L.d4, L.f4);
         CF_CHECK(hipGetLastError());
     }
     k_cell_bounds<<<(unsigned)((L.ncells + 255) / 256), 256, 0, E.stream>>>(L.ncells, L.bstart, L.x, L.y, L.z,
@@ -630,6 +631,44 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         P.slab_margin = 1e-7 * (coord_scale + g.zbs);
     }
 
+    // ---- bin lookup table: for each 1/16-octave bucket of (float)r^2, the number
+    // of edges strictly below the bucket (a lower bound of the bin index)
+    std::vector<unsigned short> lut;
+    {
+        auto bucket = [](double v) {
+            const float f = (float)v;
+            uint32_t b;
+            memcpy(&b, &f, 4);
+            return (int)(b >> 19);
+        };
+        double minpos = INFINITY;
+        for (int m = 0; m + 1 < nbin; m++)
+            if (rupp_sqr[m] > 0 && rupp_sqr[m] < minpos) minpos = rupp_sqr[m];
+        if (!std::isfinite(minpos)) minpos = P.sqr_rmax > 0 ? P.sqr_rmax : 1.0;
+        int base = bucket(minpos) - 1;
+        base = base < 0 ? 0 : base;
+        const int top = bucket(P.sqr_rmax) + 1;
+        const int n = top - base + 1;
+        lut.assign(n > 0 ? n : 1, 0);
+        for (int idx = 0; idx < n; idx++) {
+            const uint32_t bits = (uint32_t)(base + idx) << 19;
+            float lo_f;
+            memcpy(&lo_f, &bits, 4);
+            const double thr = (double)lo_f * (1.0 - 0x1p-20);
+            int cnt = 0;
+            for (int m = 0; m + 1 < nbin; m++)
+                if (rupp_sqr[m] < thr) cnt++;
+            lut[idx] = (unsigned short)cnt;
+        }
+        P.lut_base = base;
+        P.lut_n = (int)lut.size();
+        if (nbin > 65535) P.sorted_edges = 0;
+    }
+    {
+        const char* dm = getenv("CORRFUNC_AMD_DENSE");
+        P.dense_mode = dm ? atoi(dm) : 0;
+    }
+
     // ---- histograms
     unsigned long long* g_np = (unsigned long long*)E.arena.get("h_np", sizeof(uint64_t) * nslots);
     double* g_avg = (double*)E.arena.get("h_avg", sizeof(double) * nslots);
@@ -639,6 +678,8 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     CF_CHECK(hipMemsetAsync(g_avg, 0, sizeof(double) * nslots, E.stream));
     CF_CHECK(hipMemsetAsync(g_w, 0, sizeof(double) * nslots, E.stream));
     CF_CHECK(hipMemcpyAsync(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice, E.stream));
+    unsigned short* d_lut = (unsigned short*)E.arena.get("lut", sizeof(unsigned short) * lut.size());
+    CF_CHECK(hipMemcpyAsync(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice, E.stream));
     const size_t lds_hist = (size_t)nslots * 24 + (size_t)nbin * 8;
     if (lds_hist + 4096 > 64 * 1024) {
         fprintf(stderr, "Error: %lld histogram slots exceed the engine's LDS histogram capacity\n", (long long)nslots);
@@ -660,12 +701,17 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
             CF_CHECK(hipMemcpyAsync(&nitems, off + ncp, sizeof(int), hipMemcpyDeviceToHost, E.stream));
             CF_CHECK(hipStreamSynchronize(E.stream));
             WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * (nitems + 1));
-            k_emit_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, L1.bstart, off, items);
+            k_emit_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, L1.bstart, L1.bounds, L2.bounds, P,
+                                                                    off, items);
             CF_CHECK(hipGetLastError());
             int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
             CF_CHECK(hipMemsetAsync(counter, 0, sizeof(int), E.stream));
             const int threads = 256;
-            const size_t lds = lds_hist + (size_t)(threads / 64) * kQueue * 5;
+            const size_t lds = lds_hist + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)(threads / 64) * kQueue * 5;
+            if (lds > 64 * 1024) {
+                fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
+                return EXIT_FAILURE;
+            }
             int ncu = 256;
             hipDeviceProp_t prop;
             if (hipGetDeviceProperties(&prop, E.device) == hipSuccess) ncu = prop.multiProcessorCount;
@@ -676,13 +722,13 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
             if (blocks > needed) blocks = needed > 0 ? needed : 1;
             if (is2d)
                 k_pairs_tuned<1><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
-                                                                               d_edges, g_np, g_avg, g_w);
+                                                                               d_edges, d_lut, g_np, g_avg, g_w);
             else if (mode == MODE_SMU)
                 k_pairs_tuned<2><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
-                                                                               d_edges, g_np, g_avg, g_w);
+                                                                               d_edges, d_lut, g_np, g_avg, g_w);
             else
                 k_pairs_tuned<0><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
-                                                                               d_edges, g_np, g_avg, g_w);
+                                                                               d_edges, d_lut, g_np, g_avg, g_w);
             CF_CHECK(hipGetLastError());
         }
     }

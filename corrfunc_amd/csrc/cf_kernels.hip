@@ -174,7 +174,7 @@ __global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restric
                           const T* __restrict__ W, GridArgs g, const int* __restrict__ bucket,
                           const int* __restrict__ bstart, int* __restrict__ fill,
                           double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
-                          double* __restrict__ w, float4* __restrict__ f4)
+                          double* __restrict__ w, double4* __restrict__ d4, float4* __restrict__ f4)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -184,7 +184,9 @@ __global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restric
     x[pos] = xv;
     y[pos] = yv;
     z[pos] = zv;
-    if (W) w[pos] = (double)W[i];
+    const double wv = W ? (double)W[i] : 0.0;
+    if (W) w[pos] = wv;
+    d4[pos] = make_double4(xv, yv, zv, wv);
     const int c = b / kSlabs;
     const int iz = c % g.nz, ixy = c / g.nz, iy = ixy % g.ny, ix = ixy / g.ny;
     const double ox = g.xmin + ix * g.xbs, oy = g.ymin + iy * g.ybs, oz = g.zmin + iz * g.zbs;
@@ -549,57 +551,130 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 // ------------------------------------------------- tuned pair kernel
 // Work item: (cell pair, chunk of <= 64 consecutive primary particles).
 // Each wave pulls work items from a device counter until the list is exhausted.
-// Lane l holds primary particle i0+l; the secondary particles of the cell pair
-// (restricted to the z-slabs the chunk can reach) are streamed through the
-// scalar unit, since their address is uniform across the wave. An FP32 test on
-// cell-local coordinates, with a proven error margin, rejects pairs that cannot
-// be in range; the survivors are compacted into a per-wave LDS queue and decided
-// 64 at a time by the exact FP64 replay, so counts and sums equal the FP64 path.
-struct WorkItem {
-    int cp;
-    int i0, i1;
-    int pad;
-};
-
+// Lane l holds primary particle i0+l. The secondary particles of the cell pair,
+// restricted to the z-slabs the chunk can reach, are streamed through the scalar
+// unit (their address is uniform across the wave). Two paths per work item:
+//  * dense (every pair of the cell pair is inside the separation range): each
+//    (i, j) is decided in FP64 in lockstep from a 32-byte (x, y, z, w) record;
+//  * sparse: an FP32 test on cell-local coordinates, with a proven error margin,
+//    rejects pairs that cannot be in range; the survivors are compacted into a
+//    per-wave LDS queue and decided 64 at a time in FP64.
+// Either way every counted pair is decided by the reference's FP64 arithmetic.
 constexpr int kQueue = 128;
 
 // Secondary particles are read through the constant address space so that the
 // wave-uniform loads become scalar (SMEM) loads into SGPRs.
-struct __attribute__((aligned(16))) F4 {
-    float x, y, z, w;
+typedef const __attribute__((address_space(4))) float4* uniform_f4_ptr;
+typedef const __attribute__((address_space(4))) double4* uniform_d4_ptr;
+
+struct KernelLds {
+    LdsHist h;
+    const double* e;           // rupp_sqr[0..nbin-1]
+    const unsigned short* lut; // bin lookup
 };
-typedef const __attribute__((address_space(4))) F4* uniform_f4_ptr;
+
+// Number of edges e[0..nbin-2] <= r2 (the reference's top-down bin search,
+// countpairs_kernels.c.src:982-993, for non-decreasing edges): start from the
+// lookup table's lower bound for the 1/16-octave bucket of (float)r2 and step up.
+__device__ __forceinline__ int lut_kbin(const PairParams& P, const KernelLds& S, double r2)
+{
+    if (!P.sorted_edges) return find_kbin(S.e, P.nbin, 0, r2);
+    int idx = (int)(__float_as_uint((float)r2) >> 19) - P.lut_base;
+    idx = idx < 0 ? 0 : (idx >= P.lut_n ? P.lut_n - 1 : idx);
+    int k = S.lut[idx];
+    while (k < P.nbin - 1 && S.e[k] <= r2) k++;
+    return k;
+}
+
+// sqrt(r2) to ~1e-14 relative: FP32 square root refined by one Newton step in FP64.
+__device__ __forceinline__ double fast_sqrt(double r2)
+{
+    const float f = (float)r2;
+    if (!(f > 1e-30f)) return sqrt(r2);
+    const float s = __builtin_sqrtf(f);
+    const double sd = (double)s;
+    const double res = fma(-sd, sd, r2);
+    const double half_inv = 0.5 * (double)__builtin_amdgcn_rcpf(s);
+    return fma(res, half_inv, sd);
+}
+
+// Decides one (i, j) pair in FP64 exactly as the reference does and adds it to
+// the LDS histogram. (xpos, ypos, zpos) = primary position plus the cell pair's
+// wrap; (xj, yj, zj) = secondary position.
+__device__ __forceinline__ void exact_pair(const PairParams& P, int mode, const CellPairRec& cp,
+                                           const DevLattice& L1, int i, double xpos, double ypos, double zpos,
+                                           double wi, double xj, double yj, double zj, double wj, KernelLds& S)
+{
+    const double dx = xj - xpos, dy = yj - ypos, dz = zj - zpos;
+    double r2, adz = 0.0, mu = 0.0, sqr_dz = 0.0;
+    if (mode == MODE_RPPI || mode == MODE_WP) {
+        if (dz <= -P.pimax || dz >= P.pimax) return;
+        adz = fabs(dz);
+        r2 = rp2_f(P.r2form, dx, dy);
+    } else if (mode == MODE_SMU) {
+        sqr_dz = dz * dz;
+        r2 = P.r2form == R2_FMA_DX_DY_DZ ? fma(dx, dx, fma(dy, dy, sqr_dz)) : (dx * dx + dy * dy) + sqr_dz;
+    } else {
+        r2 = r2_3d(P.r2form, dx, dy, dz);
+    }
+    if (r2 >= P.sqr_rmax || r2 < P.sqr_rmin) return;
+    if (mode == MODE_SMU && sqr_dz >= r2 * P.sqr_mu_max) return;
+    // The reference's pruning tests (per-particle minimum separation, z window)
+    // can only reject pairs within a few ulps of the coordinate scale of rmax or
+    // pimax; only such pairs replay them.
+    const int near = r2 >= P.sqr_rmax - P.near_r2 ||
+                     ((mode != MODE_DD && mode != MODE_XI) && fabs(dz) >= P.pimax - P.near_dz);
+    if (near) {
+        ITerms t;
+        i_terms(P, mode, cp, L1.x[i], L1.y[i], L1.z[i], &t);
+        if (mode == MODE_RPPI || mode == MODE_WP) {
+            if (!cp.same && (t.skip || zj <= t.target)) return;
+        } else {
+            if (t.skip || zj <= t.target || dz >= t.max_dz) return;
+        }
+    }
+    const int k = lut_kbin(P, S, r2);
+    if (k == 0) return;
+    int slot;
+    if (mode == MODE_RPPI) {
+        if (P.float_index) {
+            slot = (int)((double)k * (double)(P.n2 + 1) + adz * P.inv_d2);
+        } else {
+            int pb = (int)(adz * P.inv_d2);
+            pb = pb > P.n2 ? P.n2 : pb;
+            slot = k * (P.n2 + 1) + pb;
+        }
+    } else if (mode == MODE_SMU) {
+        mu = sqrt(sqr_dz / r2);
+        if (P.float_index) {
+            slot = (int)((double)k * (double)(P.n2 + 1) + mu * P.inv_d2);
+        } else {
+            int mb = (int)(mu * P.inv_d2);
+            mb = mb > P.n2 ? P.n2 : mb;
+            slot = k * (P.n2 + 1) + mb;
+        }
+    } else {
+        slot = k;
+    }
+    atomicAdd(&S.h.np[slot], 1ull);
+    if (P.need_avg) atomicAdd(&S.h.avg[slot], fast_sqrt(r2));
+    if (P.need_w) atomicAdd(&S.h.w[slot], wi * wj);
+}
 
 __device__ __forceinline__ void drain_queue(const PairParams& P, int mode, const CellPairRec& cp,
                                             const DevLattice& L1, const DevLattice& L2, int i0,
                                             const int* qj, const unsigned char* ql, int head, int ndrain,
-                                            const double* e, LdsHist& h)
+                                            KernelLds& S)
 {
     const int lane = lane_id();
     if (lane < ndrain) {
         const int slotq = (head + lane) & (kQueue - 1);
         const int j = qj[slotq];
         const int i = i0 + ql[slotq];
-        ITerms t;
-        t.xpos = L1.x[i] + cp.wx * P.xwrap;
-        t.ypos = L1.y[i] + cp.wy * P.ywrap;
-        t.zpos = L1.z[i] + cp.wz * P.zwrap;
-        t.skip = 0;
-        t.max_dz = 0;
-        t.target = 0;
-        const double xj = L2.x[j], yj = L2.y[j], zj = L2.z[j];
-        // pairs near an edge of the z window or of the largest bin replay the
-        // reference's per-particle pruning tests; all others provably pass them
-        const double dx = xj - t.xpos, dy = yj - t.ypos, dz = zj - t.zpos;
-        double q2;
-        if (mode == MODE_RPPI || mode == MODE_WP) q2 = dx * dx + dy * dy;
-        else q2 = (dx * dx + dy * dy) + dz * dz;
-        const int near = q2 >= P.sqr_rmax - P.near_r2 ||
-                         ((mode != MODE_DD && mode != MODE_XI) && fabs(dz) >= P.pimax - P.near_dz);
-        if (near) i_terms(P, mode, cp, L1.x[i], L1.y[i], L1.z[i], &t);
-        double sep = 0;
-        const int slot = replay_pair(P, mode, t, cp.same, near, xj, yj, zj, e, &sep);
-        if (slot >= 0) hist_add(P, h, slot, sep, P.need_w ? L1.w[i] * L2.w[j] : 0.0);
+        const double4 pi = L1.d4[i];
+        const double4 pj = L2.d4[j];
+        exact_pair(P, mode, cp, L1, i, pi.x + cp.wx * P.xwrap, pi.y + cp.wy * P.ywrap, pi.z + cp.wz * P.zwrap,
+                   pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
 }
 
@@ -609,25 +684,31 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
                                                      const WorkItem* __restrict__ items, int nitems,
                                                      int* __restrict__ counter,
                                                      const double* __restrict__ rupp_sqr,
+                                                     const unsigned short* __restrict__ lut_g,
                                                      unsigned long long* __restrict__ g_np,
                                                      double* __restrict__ g_avg, double* __restrict__ g_w)
 {
     extern __shared__ unsigned long long smem_u64[];
-    LdsHist h;
-    h.np = smem_u64;
-    h.avg = reinterpret_cast<double*>(smem_u64 + P.nslots);
-    h.w = h.avg + P.nslots;
-    double* e = h.w + P.nslots;
-    int* qbase = reinterpret_cast<int*>(e + P.nbin);
+    KernelLds S;
+    S.h.np = smem_u64;
+    S.h.avg = reinterpret_cast<double*>(smem_u64 + P.nslots);
+    S.h.w = S.h.avg + P.nslots;
+    double* e = S.h.w + P.nslots;
+    unsigned short* lut = reinterpret_cast<unsigned short*>(e + P.nbin);
+    int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
+    S.e = e;
+    S.lut = lut;
     const int wave = threadIdx.x >> 6, lane = lane_id();
     int* qj = qbase + wave * kQueue;
     unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + (blockDim.x >> 6) * kQueue) + wave * kQueue;
     for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
-    hist_init(P, h);
+    for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
+    hist_init(P, S.h);
     __syncthreads();
     const int mode = P.mode;
     const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
     uniform_f4_ptr f4j = (uniform_f4_ptr)L2.f4;
+    uniform_d4_ptr d4j = (uniform_d4_ptr)L2.d4;
 
     for (;;) {
         int item = 0;
@@ -637,22 +718,20 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
         const WorkItem it = items[item];
         const CellPairRec cp = cps[__builtin_amdgcn_readfirstlane(it.cp)];
         const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
+        const int dense = P.dense_mode == 1 ? 1 : (P.dense_mode == 2 ? 0 : (__builtin_amdgcn_readfirstlane(it.flags) & kItemDense));
         const int i = i0 + lane;
         const int ivalid = i < i1;
         const int c2 = cp.c2;
-        const int iz2 = c2 % L2.nz, ixy2 = c2 / L2.nz, iy2 = ixy2 % L2.ny, ix2 = ixy2 / L2.ny;
-        const double ox = L2.xmin + ix2 * L2.xbs, oy = L2.ymin + iy2 * L2.ybs, oz = L2.zmin + iz2 * L2.zbs;
-        double zpos = 0;
-        float xr = 0.f, yr = 0.f, zr_i = 0.f;
+        double xpos = 0, ypos = 0, zpos = 0, wi = 0;
         if (ivalid) {
-            const double xpos = L1.x[i] + cp.wx * P.xwrap;
-            const double ypos = L1.y[i] + cp.wy * P.ywrap;
-            zpos = L1.z[i] + cp.wz * P.zwrap;
-            xr = (float)(xpos - ox);
-            yr = (float)(ypos - oy);
-            zr_i = (float)(zpos - oz);
+            const double4 pi = L1.d4[i];
+            xpos = pi.x + cp.wx * P.xwrap;
+            ypos = pi.y + cp.wy * P.ywrap;
+            zpos = pi.z + cp.wz * P.zwrap;
+            wi = pi.w;
         }
         // secondary z-slabs this chunk can reach
+        const int iz2 = c2 % L2.nz;
         const double zlo = wave_min_d(ivalid ? zpos : DBL_MAX) - P.zreach - P.slab_margin;
         const double zhi = wave_max_d(ivalid ? zpos : -DBL_MAX) + P.zreach + P.slab_margin;
         int jlo, jhi;
@@ -674,6 +753,18 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
         jlo = __builtin_amdgcn_readfirstlane(jlo);
         jhi = __builtin_amdgcn_readfirstlane(jhi);
 
+        if (dense) {
+            for (int j = jlo; j < jhi; j++) {
+                const double xj = d4j[j].x, yj = d4j[j].y, zj = d4j[j].z, wj = d4j[j].w;
+                if (ivalid && (!cp.same || j > i))
+                    exact_pair(P, mode, cp, L1, i, xpos, ypos, zpos, wi, xj, yj, zj, wj, S);
+            }
+            continue;
+        }
+
+        const int iz2b = c2 % L2.nz, ixy2 = c2 / L2.nz, iy2 = ixy2 % L2.ny, ix2 = ixy2 / L2.ny;
+        const double ox = L2.xmin + ix2 * L2.xbs, oy = L2.ymin + iy2 * L2.ybs, oz = L2.zmin + iz2b * L2.zbs;
+        const float xr = (float)(xpos - ox), yr = (float)(ypos - oy), zr_i = (float)(zpos - oz);
         int head = 0, tail = 0, count = 0;
         for (int j = jlo; j < jhi; j++) {
             const float pjx = f4j[j].x, pjy = f4j[j].y, pjz = f4j[j].z;
@@ -701,7 +792,7 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
                 count += added;
                 if (count >= 64) {
                     __builtin_amdgcn_wave_barrier();
-                    drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, 64, e, h);
+                    drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, 64, S);
                     __builtin_amdgcn_wave_barrier();
                     head += 64;
                     count -= 64;
@@ -710,20 +801,23 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
         }
         if (count > 0) {
             __builtin_amdgcn_wave_barrier();
-            drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, count, e, h);
+            drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, count, S);
             __builtin_amdgcn_wave_barrier();
         }
     }
     __syncthreads();
-    hist_flush(P, h, g_np, g_avg, g_w);
+    hist_flush(P, S.h, g_np, g_avg, g_w);
 }
 
 template __global__ void k_pairs_tuned<0>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
-                                          int*, const double*, unsigned long long*, double*, double*);
+                                          int*, const double*, const unsigned short*, unsigned long long*, double*,
+                                          double*);
 template __global__ void k_pairs_tuned<1>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
-                                          int*, const double*, unsigned long long*, double*, double*);
+                                          int*, const double*, const unsigned short*, unsigned long long*, double*,
+                                          double*);
 template __global__ void k_pairs_tuned<2>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
-                                          int*, const double*, unsigned long long*, double*, double*);
+                                          int*, const double*, const unsigned short*, unsigned long long*, double*,
+                                          double*);
 
 // Work-item construction: chunks of 64 primary particles per cell pair.
 __global__ void k_count_chunks(int ncp, const CellPairRec* __restrict__ cps, const int* __restrict__ bs1,
@@ -736,11 +830,26 @@ __global__ void k_count_chunks(int ncp, const CellPairRec* __restrict__ cps, con
 }
 
 __global__ void k_emit_chunks(int ncp, const CellPairRec* __restrict__ cps, const int* __restrict__ bs1,
+                              const double* __restrict__ bd1, const double* __restrict__ bd2, PairParams P,
                               const int* __restrict__ off, WorkItem* __restrict__ items)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncp) return;
-    const int c1 = cps[t].c1;
+    const CellPairRec cp = cps[t];
+    const int c1 = cp.c1;
+    // Largest separation between the two cells' bounding boxes (primary box
+    // shifted by the wrap): when it is inside the range, every pair is.
+    const double* b1 = bd1 + (int64_t)c1 * 6;
+    const double* b2 = bd2 + (int64_t)cp.c2 * 6;
+    const double ox = cp.wx * P.xwrap, oy = cp.wy * P.ywrap, oz = cp.wz * P.zwrap;
+    const double mx = fmax(fabs(b2[1] - (b1[0] + ox)), fabs((b1[1] + ox) - b2[0]));
+    const double my = fmax(fabs(b2[3] - (b1[2] + oy)), fabs((b1[3] + oy) - b2[2]));
+    const double mz = fmax(fabs(b2[5] - (b1[4] + oz)), fabs((b1[5] + oz) - b2[4]));
+    int dense;
+    if (P.mode == MODE_RPPI || P.mode == MODE_WP)
+        dense = mx * mx + my * my < 0.98 * P.sqr_rmax && mz < 0.98 * P.pimax;
+    else
+        dense = mx * mx + my * my + mz * mz < 0.98 * P.sqr_rmax;
     const int s = bs1[(int64_t)c1 * kSlabs], e = bs1[(int64_t)(c1 + 1) * kSlabs];
     int o = off[t];
     for (int i = s; i < e; i += 64) {
@@ -748,7 +857,7 @@ __global__ void k_emit_chunks(int ncp, const CellPairRec* __restrict__ cps, cons
         w.cp = t;
         w.i0 = i;
         w.i1 = i + 64 < e ? i + 64 : e;
-        w.pad = 0;
+        w.flags = dense ? kItemDense : 0;
         items[o++] = w;
     }
 }

@@ -2,6 +2,8 @@
 
 npairs must be bit-exact; ravg / savg / weightavg / xi / wp within 1e-10 relative.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -10,19 +12,28 @@ from helpers import MODES, assert_same, clustered, mode_kwargs, run, uniform, we
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = (_lib.ENGINE_TUNED, _lib.ENGINE_PLAIN)
+# engine variants: (kernel, CORRFUNC_AMD_DENSE) -- the tuned kernel picks its
+# dense or sparse path per work item; 1 / 2 force one path for every item
+ENGINES = {"tuned": (_lib.ENGINE_TUNED, "0"), "dense": (_lib.ENGINE_TUNED, "1"),
+           "sparse": (_lib.ENGINE_TUNED, "2"), "plain": (_lib.ENGINE_PLAIN, "0")}
+
+
+def _select(eng):
+    kind, dense = ENGINES[eng]
+    os.environ["CORRFUNC_AMD_DENSE"] = dense
+    _lib.set_engine(kind)
 
 
 def _opts(**kw):
     return _abi.make_options(**kw)
 
 
-@pytest.mark.parametrize("eng", ENGINES, ids=["tuned", "plain"])
+@pytest.mark.parametrize("eng", list(ENGINES))
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
 @pytest.mark.parametrize("isa", ["fallback", "avx512f"])
 def test_modes_uniform(engine, oracle, eng, mode, dtype, isa):
-    _lib.set_engine(eng)
+    _select(eng)
     L = 200.0
     X = uniform(20000, L, 11, dtype)
     W = weights(20000, 12, dtype)
@@ -37,12 +48,12 @@ def test_modes_uniform(engine, oracle, eng, mode, dtype, isa):
         assert_same(mode, got, want, what="autocorr=%d" % autocorr)
 
 
-@pytest.mark.parametrize("eng", ENGINES, ids=["tuned", "plain"])
+@pytest.mark.parametrize("eng", list(ENGINES))
 @pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_s_mu"])
 @pytest.mark.parametrize("periodic", [True, False])
 @pytest.mark.parametrize("refine", [(2, 2, 1), (1, 1, 1), (3, 3, 2)])
 def test_options_clustered(engine, oracle, eng, mode, periodic, refine):
-    _lib.set_engine(eng)
+    _select(eng)
     L = 300.0
     X = clustered(30000, L, 21, nparent=40, sigma=3.0)
     W = weights(30000, 22, np.float32)
@@ -54,13 +65,13 @@ def test_options_clustered(engine, oracle, eng, mode, periodic, refine):
     assert_same(mode, got, want)
 
 
-@pytest.mark.parametrize("eng", ENGINES, ids=["tuned", "plain"])
+@pytest.mark.parametrize("eng", list(ENGINES))
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
 def test_device_resident_inputs(engine, oracle, eng, dtype):
     """Positions and weights already in HBM are used in place (same result as host inputs)."""
     import ctypes as C
     import os
-    _lib.set_engine(eng)
+    _select(eng)
     L = 150.0
     X = uniform(10000, L, 31, dtype)
     W = weights(10000, 32, dtype)
