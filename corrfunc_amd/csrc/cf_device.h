@@ -72,7 +72,7 @@ struct PairParams {
     int lut_base;            // bucket index of lut[0]
     int lut_n;               // entries
     int dense_mode;          // 0: by work-item flag, 1: always dense, 2: never dense
-    int pad_i;
+    int hist_copies;         // LDS histogram replication R (power of 2): lane l adds to copy l % R
 };
 
 // Work item of the tuned kernel: (cell pair, <= 64 consecutive primary particles).

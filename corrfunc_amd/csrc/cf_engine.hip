@@ -707,7 +707,17 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
             int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
             CF_CHECK(hipMemsetAsync(counter, 0, sizeof(int), E.stream));
             const int threads = 256;
-            const size_t lds = lds_hist + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)(threads / 64) * kQueue * 5;
+            // histogram replication: as many copies (power of 2, <= 32) as fit a 40 KiB budget
+            const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
+            int R = 1;
+            while (R < 32 && (size_t)nslots * per_slot * (2 * R) <= 40 * 1024) R *= 2;
+            if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
+                const int v = atoi(rc);
+                if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
+            }
+            P.hist_copies = R;
+            const size_t lds = (size_t)nslots * per_slot * R + (size_t)nbin * 8 +
+                               (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)(threads / 64) * kQueue * 5;
             if (lds > 64 * 1024) {
                 fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
                 return EXIT_FAILURE;
@@ -720,15 +730,19 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
             long blocks = (long)ncu * per_cu;
             const long needed = (nitems + 3) / 4;
             if (blocks > needed) blocks = needed > 0 ? needed : 1;
-            if (is2d)
-                k_pairs_tuned<1><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
-                                                                               d_edges, d_lut, g_np, g_avg, g_w);
-            else if (mode == MODE_SMU)
-                k_pairs_tuned<2><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
-                                                                               d_edges, d_lut, g_np, g_avg, g_w);
-            else
-                k_pairs_tuned<0><<<(unsigned)blocks, threads, lds, E.stream>>>(P, L1, L2, cps, items, nitems, counter,
-                                                                               d_edges, d_lut, g_np, g_avg, g_w);
+            const void* kfn = nullptr;
+#define CF_K(M, F) if (P.mode == M && P.r2form == F) kfn = (const void*)&k_pairs_tuned<M, F>;
+            CF_K(MODE_DD, R2_SUM_XY_Z) CF_K(MODE_DD, R2_FMA_DZ_DY_DX)
+            CF_K(MODE_XI, R2_SUM_XY_Z) CF_K(MODE_XI, R2_X_SUM_YZ) CF_K(MODE_XI, R2_FMA_DZ_DY_DX)
+            CF_K(MODE_RPPI, RP_SUM) CF_K(MODE_RPPI, RP_FMA) CF_K(MODE_WP, RP_SUM) CF_K(MODE_WP, RP_FMA)
+            CF_K(MODE_SMU, R2_SUM_XY_Z) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ)
+#undef CF_K
+            if (!kfn) {
+                fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d\n", P.mode, P.r2form);
+                return EXIT_FAILURE;
+            }
+            void* kargs[] = {&P, &L1, &L2, &cps, &items, &nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w};
+            CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
             CF_CHECK(hipGetLastError());
         }
     }

@@ -567,8 +567,14 @@ constexpr int kQueue = 128;
 typedef const __attribute__((address_space(4))) float4* uniform_f4_ptr;
 typedef const __attribute__((address_space(4))) double4* uniform_d4_ptr;
 
+// LDS histogram replicated R ways with layout [slot][R]: lanes of one wave that
+// land in the same slot add to different copies, which sit in different banks.
 struct KernelLds {
-    LdsHist h;
+    unsigned long long* np;
+    double* avg;
+    double* w;
+    int rmask;                 // R - 1
+    int lane_copy;             // lane & (R - 1)
     const double* e;           // rupp_sqr[0..nbin-1]
     const unsigned short* lut; // bin lookup
 };
@@ -598,45 +604,63 @@ __device__ __forceinline__ double fast_sqrt(double r2)
     return fma(res, half_inv, sd);
 }
 
+// FP64 separation in the rounding order R2F of the reference kernel being replayed.
+template <int R2F>
+__device__ __forceinline__ double sep2(double dx, double dy, double dz)
+{
+    if (R2F == R2_FMA_DZ_DY_DX) return fma(dz, dz, fma(dy, dy, dx * dx));
+    if (R2F == R2_X_SUM_YZ) return dx * dx + (dy * dy + dz * dz);
+    if (R2F == R2_FMA_DX_DY_DZ) return fma(dx, dx, fma(dy, dy, dz * dz));
+    if (R2F == RP_FMA) return fma(dy, dy, dx * dx);
+    if (R2F == RP_SUM) return dx * dx + dy * dy;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
 // Decides one (i, j) pair in FP64 exactly as the reference does and adds it to
 // the LDS histogram. (xpos, ypos, zpos) = primary position plus the cell pair's
-// wrap; (xj, yj, zj) = secondary position.
-__device__ __forceinline__ void exact_pair(const PairParams& P, int mode, const CellPairRec& cp,
-                                           const DevLattice& L1, int i, double xpos, double ypos, double zpos,
-                                           double wi, double xj, double yj, double zj, double wj, KernelLds& S)
+// wrap; (xj, yj, zj) = secondary position. `ok` masks lanes without a pair.
+template <int MODE, int R2F>
+__device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRec& cp, const DevLattice& L1, int i,
+                                           int ok, double xpos, double ypos, double zpos, double wi, double xj,
+                                           double yj, double zj, double wj, KernelLds& S)
 {
+    constexpr bool is2d = MODE == MODE_RPPI || MODE == MODE_WP;
     const double dx = xj - xpos, dy = yj - ypos, dz = zj - zpos;
-    double r2, adz = 0.0, mu = 0.0, sqr_dz = 0.0;
-    if (mode == MODE_RPPI || mode == MODE_WP) {
-        if (dz <= -P.pimax || dz >= P.pimax) return;
-        adz = fabs(dz);
-        r2 = rp2_f(P.r2form, dx, dy);
-    } else if (mode == MODE_SMU) {
-        sqr_dz = dz * dz;
-        r2 = P.r2form == R2_FMA_DX_DY_DZ ? fma(dx, dx, fma(dy, dy, sqr_dz)) : (dx * dx + dy * dy) + sqr_dz;
-    } else {
-        r2 = r2_3d(P.r2form, dx, dy, dz);
-    }
-    if (r2 >= P.sqr_rmax || r2 < P.sqr_rmin) return;
-    if (mode == MODE_SMU && sqr_dz >= r2 * P.sqr_mu_max) return;
+    const double sqr_dz = dz * dz;
+    const double r2 = sep2<R2F>(dx, dy, dz);
+    const double adz = fabs(dz);
+    bool in = ok && r2 < P.sqr_rmax && r2 >= P.sqr_rmin;
+    if (is2d) in = in && dz > -P.pimax && dz < P.pimax;
+    if (MODE == MODE_SMU) in = in && sqr_dz < r2 * P.sqr_mu_max;
+    if (!in) return;
     // The reference's pruning tests (per-particle minimum separation, z window)
     // can only reject pairs within a few ulps of the coordinate scale of rmax or
     // pimax; only such pairs replay them.
-    const int near = r2 >= P.sqr_rmax - P.near_r2 ||
-                     ((mode != MODE_DD && mode != MODE_XI) && fabs(dz) >= P.pimax - P.near_dz);
+    const bool near = r2 >= P.sqr_rmax - P.near_r2 || (MODE != MODE_DD && MODE != MODE_XI && adz >= P.pimax - P.near_dz);
     if (near) {
         ITerms t;
-        i_terms(P, mode, cp, L1.x[i], L1.y[i], L1.z[i], &t);
-        if (mode == MODE_RPPI || mode == MODE_WP) {
+        i_terms(P, MODE, cp, L1.x[i], L1.y[i], L1.z[i], &t);
+        if (is2d) {
             if (!cp.same && (t.skip || zj <= t.target)) return;
         } else {
             if (t.skip || zj <= t.target || dz >= t.max_dz) return;
         }
     }
-    const int k = lut_kbin(P, S, r2);
+    int k;
+    if (P.sorted_edges) {
+        int idx = (int)(__float_as_uint((float)r2) >> 19) - P.lut_base;
+        idx = idx < 0 ? 0 : (idx >= P.lut_n ? P.lut_n - 1 : idx);
+        k = S.lut[idx];
+        if (k < P.nbin - 1 && S.e[k] <= r2) {
+            k++;
+            while (k < P.nbin - 1 && S.e[k] <= r2) k++;
+        }
+    } else {
+        k = find_kbin(S.e, P.nbin, 0, r2);
+    }
     if (k == 0) return;
-    int slot;
-    if (mode == MODE_RPPI) {
+    int slot = k;
+    if (MODE == MODE_RPPI) {
         if (P.float_index) {
             slot = (int)((double)k * (double)(P.n2 + 1) + adz * P.inv_d2);
         } else {
@@ -644,8 +668,8 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, int mode, const 
             pb = pb > P.n2 ? P.n2 : pb;
             slot = k * (P.n2 + 1) + pb;
         }
-    } else if (mode == MODE_SMU) {
-        mu = sqrt(sqr_dz / r2);
+    } else if (MODE == MODE_SMU) {
+        const double mu = sqrt(sqr_dz / r2);
         if (P.float_index) {
             slot = (int)((double)k * (double)(P.n2 + 1) + mu * P.inv_d2);
         } else {
@@ -653,18 +677,17 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, int mode, const 
             mb = mb > P.n2 ? P.n2 : mb;
             slot = k * (P.n2 + 1) + mb;
         }
-    } else {
-        slot = k;
     }
-    atomicAdd(&S.h.np[slot], 1ull);
-    if (P.need_avg) atomicAdd(&S.h.avg[slot], fast_sqrt(r2));
-    if (P.need_w) atomicAdd(&S.h.w[slot], wi * wj);
+    const int a = slot * (S.rmask + 1) + S.lane_copy;
+    atomicAdd(&S.np[a], 1ull);
+    if (P.need_avg) atomicAdd(&S.avg[a], fast_sqrt(r2));
+    if (P.need_w) atomicAdd(&S.w[a], wi * wj);
 }
 
-__device__ __forceinline__ void drain_queue(const PairParams& P, int mode, const CellPairRec& cp,
-                                            const DevLattice& L1, const DevLattice& L2, int i0,
-                                            const int* qj, const unsigned char* ql, int head, int ndrain,
-                                            KernelLds& S)
+template <int MODE, int R2F>
+__device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const DevLattice& L1,
+                                            const DevLattice& L2, int i0, const int* qj, const unsigned char* ql,
+                                            int head, int ndrain, KernelLds& S)
 {
     const int lane = lane_id();
     if (lane < ndrain) {
@@ -673,12 +696,12 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, int mode, const
         const int i = i0 + ql[slotq];
         const double4 pi = L1.d4[i];
         const double4 pj = L2.d4[j];
-        exact_pair(P, mode, cp, L1, i, pi.x + cp.wx * P.xwrap, pi.y + cp.wy * P.ywrap, pi.z + cp.wz * P.zwrap,
-                   pi.w, pj.x, pj.y, pj.z, pj.w, S);
+        exact_pair<MODE, R2F>(P, cp, L1, i, 1, pi.x + cp.wx * P.xwrap, pi.y + cp.wy * P.ywrap,
+                              pi.z + cp.wz * P.zwrap, pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
 }
 
-template <int MODEK>  // 0: DD/xi (3-D r), 1: rp_pi/wp, 2: s_mu
+template <int MODE, int R2F>
 __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1, DevLattice L2,
                                                      const CellPairRec* __restrict__ cps,
                                                      const WorkItem* __restrict__ items, int nitems,
@@ -690,10 +713,14 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
 {
     extern __shared__ unsigned long long smem_u64[];
     KernelLds S;
-    S.h.np = smem_u64;
-    S.h.avg = reinterpret_cast<double*>(smem_u64 + P.nslots);
-    S.h.w = S.h.avg + P.nslots;
-    double* e = S.h.w + P.nslots;
+    const int R = P.hist_copies;
+    const int nrep = P.nslots * R;
+    S.np = smem_u64;
+    S.avg = reinterpret_cast<double*>(smem_u64 + nrep);
+    S.w = S.avg + (P.need_avg ? nrep : 0);
+    double* e = S.w + (P.need_w ? nrep : 0);
+    S.rmask = R - 1;
+    S.lane_copy = lane_id() & (R - 1);
     unsigned short* lut = reinterpret_cast<unsigned short*>(e + P.nbin);
     int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
     S.e = e;
@@ -703,9 +730,12 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
     unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + (blockDim.x >> 6) * kQueue) + wave * kQueue;
     for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
     for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
-    hist_init(P, S.h);
+    for (int s = threadIdx.x; s < nrep; s += blockDim.x) {
+        S.np[s] = 0;
+        if (P.need_avg) S.avg[s] = 0;
+        if (P.need_w) S.w[s] = 0;
+    }
     __syncthreads();
-    const int mode = P.mode;
     const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
     uniform_f4_ptr f4j = (uniform_f4_ptr)L2.f4;
     uniform_d4_ptr d4j = (uniform_d4_ptr)L2.d4;
@@ -754,10 +784,11 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
         jhi = __builtin_amdgcn_readfirstlane(jhi);
 
         if (dense) {
+            const int same = cp.same;
             for (int j = jlo; j < jhi; j++) {
                 const double xj = d4j[j].x, yj = d4j[j].y, zj = d4j[j].z, wj = d4j[j].w;
-                if (ivalid && (!cp.same || j > i))
-                    exact_pair(P, mode, cp, L1, i, xpos, ypos, zpos, wi, xj, yj, zj, wj, S);
+                exact_pair<MODE, R2F>(P, cp, L1, i, ivalid && (!same || j > i), xpos, ypos, zpos, wi, xj, yj, zj,
+                                      wj, S);
             }
             continue;
         }
@@ -771,7 +802,7 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
             const float dx = pjx - xr, dy = pjy - yr, dz = pjz - zr_i;
             float q;
             int pass;
-            if (MODEK == 1) {
+            if (MODE == MODE_RPPI || MODE == MODE_WP) {
                 q = fmaf(dy, dy, dx * dx);
                 pass = ivalid && q < hi2 && q >= lo2 && fabsf(dz) < zr;
             } else {
@@ -792,7 +823,7 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
                 count += added;
                 if (count >= 64) {
                     __builtin_amdgcn_wave_barrier();
-                    drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, 64, S);
+                    drain_queue<MODE, R2F>(P, cp, L1, L2, i0, qj, ql, head, 64, S);
                     __builtin_amdgcn_wave_barrier();
                     head += 64;
                     count -= 64;
@@ -801,23 +832,49 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
         }
         if (count > 0) {
             __builtin_amdgcn_wave_barrier();
-            drain_queue(P, mode, cp, L1, L2, i0, qj, ql, head, count, S);
+            drain_queue<MODE, R2F>(P, cp, L1, L2, i0, qj, ql, head, count, S);
             __builtin_amdgcn_wave_barrier();
         }
     }
     __syncthreads();
-    hist_flush(P, S.h, g_np, g_avg, g_w);
+    for (int sl = threadIdx.x; sl < P.nslots; sl += blockDim.x) {
+        unsigned long long c = 0;
+        double sa = 0, sw = 0;
+        for (int r = 0; r < R; r++) {
+            c += S.np[sl * R + r];
+            if (P.need_avg) sa += S.avg[sl * R + r];
+            if (P.need_w) sw += S.w[sl * R + r];
+        }
+        if (c) {
+            atomicAdd(&g_np[sl], c);
+            if (P.need_avg) atomicAdd(&g_avg[sl], sa);
+            if (P.need_w) atomicAdd(&g_w[sl], sw);
+        }
+    }
 }
 
-template __global__ void k_pairs_tuned<0>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
-                                          int*, const double*, const unsigned short*, unsigned long long*, double*,
-                                          double*);
-template __global__ void k_pairs_tuned<1>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
-                                          int*, const double*, const unsigned short*, unsigned long long*, double*,
-                                          double*);
-template __global__ void k_pairs_tuned<2>(PairParams, DevLattice, DevLattice, const CellPairRec*, const WorkItem*, int,
-                                          int*, const double*, const unsigned short*, unsigned long long*, double*,
-                                          double*);
+template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_XI, R2_SUM_XY_Z>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_XI, R2_X_SUM_YZ>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_XI, R2_FMA_DZ_DY_DX>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_RPPI, RP_SUM>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_RPPI, RP_FMA>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_WP, RP_SUM>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_WP, RP_FMA>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_SMU, R2_SUM_XY_Z>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_SMU, R2_FMA_DX_DY_DZ>(PairParams, DevLattice, DevLattice, const CellPairRec*,
+    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
 
 // Work-item construction: chunks of 64 primary particles per cell pair.
 __global__ void k_count_chunks(int ncp, const CellPairRec* __restrict__ cps, const int* __restrict__ bs1,
