@@ -46,6 +46,11 @@ def lib():
         L.cf_amd_device_free.restype = C.c_int
         L.cf_amd_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         L.cf_amd_memcpy.restype = C.c_int
+        L.cf_amd_countpairs_batch.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_void_p),
+                                              C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                              C.c_int, C.c_char_p, C.POINTER(_abi.results_countpairs),
+                                              C.POINTER(_abi.config_options), C.POINTER(_abi.extra_options)]
+        L.cf_amd_countpairs_batch.restype = C.c_int
         _lib = L
     return _lib
 
@@ -100,3 +105,50 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def countpairs_batch(catalogs, binfile, options=None, weights=None, nthreads=1):
+    """DD(r) auto-correlations of many catalogs in one device launch.
+
+    catalogs: sequence of (X, Y, Z) arrays (numpy, or DeviceArray for HBM-resident
+    data); weights: optional sequence of per-catalog weight arrays (pair_product).
+    Returns a list of dicts with the countpairs outputs of each catalog."""
+    import os
+    from . import _abi
+    ncat = len(catalogs)
+    keep = []
+
+    def ptr(a):
+        if isinstance(a, DeviceArray):
+            return a.ptr
+        keep.append(a)
+        return a.ctypes.data
+
+    ND = (C.c_int64 * ncat)()
+    Xs, Ys, Zs, Ws = [(C.c_void_p * ncat)() for _ in range(4)]
+    dtype = None
+    for k, (x, y, z) in enumerate(catalogs):
+        ND[k] = x.shape[0]
+        Xs[k], Ys[k], Zs[k] = ptr(x), ptr(y), ptr(z)
+        dtype = x.dtype
+        if weights is not None:
+            Ws[k] = ptr(weights[k])
+    o = options if options is not None else _abi.make_options()
+    o.float_type = dtype.itemsize
+    extra = _abi.get_extra_options(_abi.WEIGHT_PAIR_PRODUCT if weights is not None else _abi.WEIGHT_NONE)
+    res = (_abi.results_countpairs * ncat)()
+    path, tmp = _abi.write_bin_file(binfile)
+    try:
+        st = lib().cf_amd_countpairs_batch(ncat, ND, Xs, Ys, Zs, Ws, int(nthreads), path.encode(), res,
+                                           C.byref(o), C.byref(extra))
+    finally:
+        if tmp:
+            os.remove(path)
+    if st != 0:
+        raise RuntimeError("cf_amd_countpairs_batch failed")
+    out = []
+    for k in range(ncat):
+        d = _abi._collect("countpairs", res[k])
+        lib().free_results(C.byref(res[k]))
+        out.append(d)
+    return out

@@ -73,14 +73,17 @@ struct PairParams {
     int lut_n;               // entries
     int dense_mode;          // 0: by work-item flag, 1: always dense, 2: never dense
     int hist_copies;         // LDS histogram replication R (power of 2): lane l adds to copy l % R
+    int dense_stride;        // 1: dense work items take strided primaries
+    int pad_i2;
 };
 
 // Work item of the tuned kernel: (cell pair, <= 64 consecutive primary particles).
 struct WorkItem {
     int cp;
-    int i0, i1;
-    int flags;  // bit 0: every pair of the cell pair lies inside the separation range
+    int i0, i1;  // contiguous items: primaries [i0, i1); strided: i0 + lane*stride < i1
+    int flags;   // bit 0: dense (every pair of the cell pair inside the range); bits 8..: stride
 };
 constexpr int kItemDense = 1;
+constexpr int kStrideShift = 8;
 
 }  // namespace cfamd

@@ -49,6 +49,8 @@ struct Arena {
         auto it = slots.find(name);
         if (it != slots.end() && it->second.second >= bytes) return it->second.first;
         if (it != slots.end()) {
+            // queued work may still read the old buffer
+            CF_CHECK(hipDeviceSynchronize());
             CF_CHECK(hipFree(it->second.first));
             slots.erase(it);
         }
@@ -310,7 +312,7 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
 // Device work list of neighbour-cell pairs (gridlink_impl.c.src:439-625).
 static int cellpairs_device(const DevLattice& L1, const DevLattice& L2, const int refine[3], const int per[3],
                             const double wrap[3], double max3, double max2, double max1, int min_sep_opt,
-                            int autocorr, CellPairRec** out, int* ncp_out, double* tested)
+                            int autocorr, CellPairRec** out, int* ncp_out, double* tested, int cat = 0)
 {
     Engine& E = eng();
     NgbArgs a;
@@ -343,7 +345,7 @@ static int cellpairs_device(const DevLattice& L1, const DevLattice& L2, const in
     CellPairRec* cps = (CellPairRec*)E.arena.get("cp_list", sizeof(CellPairRec) * (ncp + 1));
     CF_CHECK(hipMemsetAsync(dtested, 0, sizeof(double), E.stream));
     k_cellpair_emit<<<blocks, 256, 0, E.stream>>>(a, L1.ncells, nngb, L1.bstart, L2.bstart, L1.bounds, L2.bounds,
-                                                  pos, cps, dtested);
+                                                  pos, cps, dtested, cat);
     CF_CHECK(hipGetLastError());
     CF_CHECK(hipMemcpyAsync(tested, dtested, sizeof(double), hipMemcpyDeviceToHost, E.stream));
     CF_CHECK(hipStreamSynchronize(E.stream));
@@ -402,6 +404,151 @@ static int r2form_for(int mode, int isa_id)
 
 static inline float f32_up(double v) { return nextafterf((float)v, INFINITY); }
 static inline float f32_down(double v) { return nextafterf((float)v, -INFINITY); }
+
+// Refine heuristics, grid, boost (countpairs_impl.c.src:272-332 and the per-mode
+// variants: rp_pi_impl:277-329, s_mu_impl:276-340, xi_impl:212-270, wp_impl:240-290).
+// Updates o->bin_refine_factors as the reference does.
+static int choose_grid(int mode, int64_t N1, double rmax, double pimax, double boxsize, const double ext[6],
+                       const double wrap[3], struct config_options* o, GridSpec& g, int refine[3])
+{
+    const int is2d = mode == MODE_RPPI || mode == MODE_WP;
+    if (get_bin_refine_scheme(o) == BINNING_DFL) {
+        if (mode == MODE_DD) {
+            for (int d = 0; d < 3; d++)
+                if (rmax < 0.05 * wrap[d]) o->bin_refine_factors[d] = 1;
+        } else if (mode == MODE_RPPI) {
+            if (rmax < 0.05 * wrap[0]) o->bin_refine_factors[0] = 1;
+            if (rmax < 0.05 * wrap[1]) o->bin_refine_factors[1] = 1;
+            if (pimax < 0.05 * wrap[2]) o->bin_refine_factors[2] = 1;
+        } else if (mode == MODE_XI) {
+            if (rmax < 0.05 * boxsize)
+                for (int d = 0; d < 3; d++) o->bin_refine_factors[d] = 1;
+        } else if (mode == MODE_WP) {
+            if (rmax < 0.05 * boxsize) o->bin_refine_factors[0] = o->bin_refine_factors[1] = 1;
+            if (pimax < 0.05 * boxsize) o->bin_refine_factors[2] = 1;
+        }
+    }
+    double gsz[3] = {rmax, rmax, rmax};
+    if (is2d || mode == MODE_SMU) gsz[2] = pimax;
+    refine[0] = o->bin_refine_factors[0];
+    refine[1] = o->bin_refine_factors[1];
+    refine[2] = o->bin_refine_factors[2];
+    if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+    // boost (countpairs_impl.c.src:298-332; DDsmu variant s_mu_impl:313-340)
+    if (mode == MODE_SMU) {
+        if (g.nx <= 10 && g.ny <= 10 && g.nz <= 10 && o->max_cells_per_dim >= 10 &&
+            get_bin_refine_scheme(o) == BINNING_DFL) {
+            for (int d = 0; d < 3; d++) o->bin_refine_factors[d] *= 1;
+        }
+    } else {
+        const double avg_np = ((double)N1) / ((double)g.nx * g.ny * g.nz);
+        const int max_nmesh = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
+        if ((max_nmesh <= 10 || avg_np >= 250) && max_nmesh < o->max_cells_per_dim &&
+            get_bin_refine_scheme(o) == BINNING_DFL) {
+            o->bin_refine_factors[0] += 1;
+            o->bin_refine_factors[1] += 1;
+        }
+    }
+    refine[0] = o->bin_refine_factors[0];
+    refine[1] = o->bin_refine_factors[1];
+    refine[2] = o->bin_refine_factors[2];
+    if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+
+    return EXIT_SUCCESS;
+}
+
+// Launch parameters of the tuned kernel: rounding order, cut values, edge
+// windows, FP32 filter margins and the bin lookup table.
+static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, const std::vector<double>& rupp_sqr,
+                         int sorted, double rmax, double pimax, double inv_d2, double sqr_mu_max, int need_avg,
+                         int need_w, int autocorr, const double wrap[3], double coord_scale, double cs, int rmaxr,
+                         PairParams& P, std::vector<unsigned short>& lut)
+{
+    memset(&P, 0, sizeof P);
+    const int is2d = mode == MODE_RPPI || mode == MODE_WP;
+    P.mode = mode;
+    P.r2form = r2form_for(mode, isa_id);
+    P.float_index = (isa_id == AVX512F) && (mode == MODE_RPPI || mode == MODE_SMU);
+    P.nbin = nbin;
+    P.n2 = n2;
+    P.nslots = nslots;
+    P.need_avg = need_avg;
+    P.need_w = need_w;
+    P.autocorr = autocorr;
+    P.sorted_edges = sorted;
+    P.sqr_rmax = rupp_sqr[nbin - 1];
+    P.sqr_rmin = rupp_sqr[0];
+    P.pimax = pimax;
+    P.inv_d2 = inv_d2;
+    P.sqr_mu_max = sqr_mu_max;
+    if (mode == MODE_SMU) {
+        const double a = sqrt(P.sqr_rmax - 0.0 - 0.0);
+        P.max_all_dz_same = pimax < a ? pimax : a;
+    } else {
+        P.max_all_dz_same = sqrt(P.sqr_rmax - 0.0 - 0.0);
+    }
+    P.xwrap = wrap[0];
+    P.ywrap = wrap[1];
+    P.zwrap = wrap[2];
+    // Edge windows: the reference's pruning tests can only reject pairs whose
+    // separation lies within a few ulps of the coordinate scale below rmax / pimax.
+    P.near_r2 = 1e-9 * P.sqr_rmax + 1e-9 * rmax * coord_scale;
+    P.near_dz = 1e-9 * pimax + 1e-9 * coord_scale;
+    // FP32 filter margin: |dx_f32 - dx| <= eps for cell-local coordinates.
+    {
+        const double B = (rmaxr + 3) * cs + rmax + pimax;
+        const double eps = 8.0 * B * 0x1p-24;
+        const double r_hi = sqrt(P.sqr_rmax), r_lo = sqrt(P.sqr_rmin);
+        const double m_hi = 2.0 * sqrt(3.0) * r_hi * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmax;
+        const double m_lo = 2.0 * sqrt(3.0) * r_lo * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmin;
+        P.hi2_f = f32_up(P.sqr_rmax + m_hi);
+        P.lo2_f = P.sqr_rmin - m_lo > 0 ? f32_down(P.sqr_rmin - m_lo) : -INFINITY;
+        P.zreach_f = f32_up(pimax + 4.0 * eps);
+        P.zreach = (is2d || mode == MODE_SMU) ? pimax : sqrt(P.sqr_rmax);
+        P.slab_margin = 1e-7 * (coord_scale + cs);
+    }
+
+    // ---- bin lookup table: for each 1/16-octave bucket of (float)r^2, the number
+    // of edges strictly below the bucket (a lower bound of the bin index)
+    lut.clear();
+    {
+        auto bucket = [](double v) {
+            const float f = (float)v;
+            uint32_t b;
+            memcpy(&b, &f, 4);
+            return (int)(b >> 19);
+        };
+        double minpos = INFINITY;
+        for (int m = 0; m + 1 < nbin; m++)
+            if (rupp_sqr[m] > 0 && rupp_sqr[m] < minpos) minpos = rupp_sqr[m];
+        if (!std::isfinite(minpos)) minpos = P.sqr_rmax > 0 ? P.sqr_rmax : 1.0;
+        int base = bucket(minpos) - 1;
+        base = base < 0 ? 0 : base;
+        const int top = bucket(P.sqr_rmax) + 1;
+        const int n = top - base + 1;
+        lut.assign(n > 0 ? n : 1, 0);
+        for (int idx = 0; idx < n; idx++) {
+            const uint32_t bits = (uint32_t)(base + idx) << 19;
+            float lo_f;
+            memcpy(&lo_f, &bits, 4);
+            const double thr = (double)lo_f * (1.0 - 0x1p-20);
+            int cnt = 0;
+            for (int m = 0; m + 1 < nbin; m++)
+                if (rupp_sqr[m] < thr) cnt++;
+            lut[idx] = (unsigned short)cnt;
+        }
+        P.lut_base = base;
+        P.lut_n = (int)lut.size();
+        if (nbin > 65535) P.sorted_edges = 0;
+    }
+    {
+        const char* dm = getenv("CORRFUNC_AMD_DENSE");
+        P.dense_mode = dm ? atoi(dm) : 0;
+        const char* ds = getenv("CORRFUNC_AMD_STRIDE");
+        P.dense_stride = ds ? atoi(ds) : 1;
+    }
+
+}
 
 static int run_core(const CallSpec& c, struct config_options* o, struct extra_options* extra, CallOut& R)
 {
@@ -516,47 +663,10 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         wrap[2] = per[2] ? (bz > 0 ? bz : (ext[5] - ext[2])) : 0.;
     }
 
-    // ---- refine heuristics (countpairs_impl.c.src:272-282 and per-mode variants)
-    if (get_bin_refine_scheme(o) == BINNING_DFL) {
-        if (mode == MODE_DD) {
-            for (int d = 0; d < 3; d++)
-                if (rmax < 0.05 * wrap[d]) o->bin_refine_factors[d] = 1;
-        } else if (mode == MODE_RPPI) {
-            if (rmax < 0.05 * wrap[0]) o->bin_refine_factors[0] = 1;
-            if (rmax < 0.05 * wrap[1]) o->bin_refine_factors[1] = 1;
-            if (pimax < 0.05 * wrap[2]) o->bin_refine_factors[2] = 1;
-        } else if (mode == MODE_XI) {
-            if (rmax < 0.05 * c.boxsize)
-                for (int d = 0; d < 3; d++) o->bin_refine_factors[d] = 1;
-        } else if (mode == MODE_WP) {
-            if (rmax < 0.05 * c.boxsize) o->bin_refine_factors[0] = o->bin_refine_factors[1] = 1;
-            if (pimax < 0.05 * c.boxsize) o->bin_refine_factors[2] = 1;
-        }
-    }
-    double gsz[3] = {rmax, rmax, rmax};
-    if (is2d || mode == MODE_SMU) gsz[2] = pimax;
-    int refine[3] = {o->bin_refine_factors[0], o->bin_refine_factors[1], o->bin_refine_factors[2]};
+    // ---- grid
+    int refine[3];
     GridSpec g;
-    if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
-    // boost (countpairs_impl.c.src:298-332; DDsmu variant s_mu_impl:313-340)
-    if (mode == MODE_SMU) {
-        if (g.nx <= 10 && g.ny <= 10 && g.nz <= 10 && o->max_cells_per_dim >= 10 &&
-            get_bin_refine_scheme(o) == BINNING_DFL) {
-            for (int d = 0; d < 3; d++) o->bin_refine_factors[d] *= 1;
-        }
-    } else {
-        const double avg_np = ((double)c.N1) / ((double)g.nx * g.ny * g.nz);
-        const int max_nmesh = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
-        if ((max_nmesh <= 10 || avg_np >= 250) && max_nmesh < o->max_cells_per_dim &&
-            get_bin_refine_scheme(o) == BINNING_DFL) {
-            o->bin_refine_factors[0] += 1;
-            o->bin_refine_factors[1] += 1;
-        }
-    }
-    refine[0] = o->bin_refine_factors[0];
-    refine[1] = o->bin_refine_factors[1];
-    refine[2] = o->bin_refine_factors[2];
-    if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (choose_grid(mode, c.N1, rmax, pimax, c.boxsize, ext, wrap, o, g, refine) != EXIT_SUCCESS) return EXIT_FAILURE;
 
     // ---- gridding on the GPU
     DevLattice L1, L2;
@@ -582,91 +692,15 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
 
     // ---- launch parameters
     PairParams P;
-    memset(&P, 0, sizeof P);
-    const int isa_id = resolve_isa(o->instruction_set);
-    P.mode = mode;
-    P.r2form = r2form_for(mode, isa_id);
-    P.float_index = (isa_id == AVX512F) && (mode == MODE_RPPI || mode == MODE_SMU);
-    P.nbin = nbin;
-    P.n2 = n2;
-    P.nslots = (int)nslots;
-    P.need_avg = o->need_avg_sep;
-    P.need_w = need_w;
-    P.autocorr = c.autocorr;
-    P.sorted_edges = sorted;
-    P.sqr_rmax = rupp_sqr[nbin - 1];
-    P.sqr_rmin = rupp_sqr[0];
-    P.pimax = pimax;
-    P.inv_d2 = inv_d2;
-    P.sqr_mu_max = sqr_mu_max;
-    if (mode == MODE_SMU) {
-        const double a = sqrt(P.sqr_rmax - 0.0 - 0.0);
-        P.max_all_dz_same = pimax < a ? pimax : a;
-    } else {
-        P.max_all_dz_same = sqrt(P.sqr_rmax - 0.0 - 0.0);
-    }
-    P.xwrap = wrap[0];
-    P.ywrap = wrap[1];
-    P.zwrap = wrap[2];
-    // Edge windows: the reference's pruning tests can only reject pairs whose
-    // separation lies within a few ulps of the coordinate scale below rmax / pimax.
-    const double coord_scale = fabs(ext[0]) + fabs(ext[1]) + fabs(ext[2]) + fabs(ext[3]) + fabs(ext[4]) +
-                               fabs(ext[5]) + wrap[0] + wrap[1] + wrap[2] + rmax;
-    P.near_r2 = 1e-9 * P.sqr_rmax + 1e-9 * rmax * coord_scale;
-    P.near_dz = 1e-9 * pimax + 1e-9 * coord_scale;
-    // FP32 filter margin: |dx_f32 - dx| <= eps for cell-local coordinates.
+    std::vector<unsigned short> lut;
     {
+        const double coord_scale = fabs(ext[0]) + fabs(ext[1]) + fabs(ext[2]) + fabs(ext[3]) + fabs(ext[4]) +
+                                   fabs(ext[5]) + wrap[0] + wrap[1] + wrap[2] + rmax;
         const double cs = fmax(g.xbs, fmax(g.ybs, g.zbs));
         const int rmaxr = refine[0] > refine[1] ? (refine[0] > refine[2] ? refine[0] : refine[2])
                                                 : (refine[1] > refine[2] ? refine[1] : refine[2]);
-        const double B = (rmaxr + 3) * cs + rmax + pimax;
-        const double eps = 8.0 * B * 0x1p-24;
-        const double r_hi = sqrt(P.sqr_rmax), r_lo = sqrt(P.sqr_rmin);
-        const double m_hi = 2.0 * sqrt(3.0) * r_hi * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmax;
-        const double m_lo = 2.0 * sqrt(3.0) * r_lo * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmin;
-        P.hi2_f = f32_up(P.sqr_rmax + m_hi);
-        P.lo2_f = P.sqr_rmin - m_lo > 0 ? f32_down(P.sqr_rmin - m_lo) : -INFINITY;
-        P.zreach_f = f32_up(pimax + 4.0 * eps);
-        P.zreach = (is2d || mode == MODE_SMU) ? pimax : sqrt(P.sqr_rmax);
-        P.slab_margin = 1e-7 * (coord_scale + g.zbs);
-    }
-
-    // ---- bin lookup table: for each 1/16-octave bucket of (float)r^2, the number
-    // of edges strictly below the bucket (a lower bound of the bin index)
-    std::vector<unsigned short> lut;
-    {
-        auto bucket = [](double v) {
-            const float f = (float)v;
-            uint32_t b;
-            memcpy(&b, &f, 4);
-            return (int)(b >> 19);
-        };
-        double minpos = INFINITY;
-        for (int m = 0; m + 1 < nbin; m++)
-            if (rupp_sqr[m] > 0 && rupp_sqr[m] < minpos) minpos = rupp_sqr[m];
-        if (!std::isfinite(minpos)) minpos = P.sqr_rmax > 0 ? P.sqr_rmax : 1.0;
-        int base = bucket(minpos) - 1;
-        base = base < 0 ? 0 : base;
-        const int top = bucket(P.sqr_rmax) + 1;
-        const int n = top - base + 1;
-        lut.assign(n > 0 ? n : 1, 0);
-        for (int idx = 0; idx < n; idx++) {
-            const uint32_t bits = (uint32_t)(base + idx) << 19;
-            float lo_f;
-            memcpy(&lo_f, &bits, 4);
-            const double thr = (double)lo_f * (1.0 - 0x1p-20);
-            int cnt = 0;
-            for (int m = 0; m + 1 < nbin; m++)
-                if (rupp_sqr[m] < thr) cnt++;
-            lut[idx] = (unsigned short)cnt;
-        }
-        P.lut_base = base;
-        P.lut_n = (int)lut.size();
-        if (nbin > 65535) P.sorted_edges = 0;
-    }
-    {
-        const char* dm = getenv("CORRFUNC_AMD_DENSE");
-        P.dense_mode = dm ? atoi(dm) : 0;
+        build_params(mode, resolve_isa(o->instruction_set), nbin, n2, (int)nslots, rupp_sqr, sorted, rmax, pimax,
+                     inv_d2, sqr_mu_max, o->need_avg_sep, need_w, c.autocorr, wrap, coord_scale, cs, rmaxr, P, lut);
     }
 
     // ---- histograms
@@ -693,19 +727,22 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
             k_pairs_plain<<<ncp, 128, lds_hist, E.stream>>>(P, L1, L2, cps, d_edges, g_np, g_avg, g_w);
             CF_CHECK(hipGetLastError());
         } else {
+            DevLattice hl[2] = {L1, L2};
+            DevLattice* lats = (DevLattice*)E.arena.get("lats", sizeof(hl));
+            CF_CHECK(hipMemcpyAsync(lats, hl, sizeof(hl), hipMemcpyHostToDevice, E.stream));
             int* nch = (int*)E.arena.get("wi_n", sizeof(int) * (ncp + 1));
             int* off = (int*)E.arena.get("wi_off", sizeof(int) * (ncp + 1));
-            k_count_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, L1.bstart, nch);
+            k_count_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, lats, 0, nch);
             CF_CHECK(hipGetLastError());
             device_scan(nch, off, ncp, "wi_scan");
             CF_CHECK(hipMemcpyAsync(&nitems, off + ncp, sizeof(int), hipMemcpyDeviceToHost, E.stream));
             CF_CHECK(hipStreamSynchronize(E.stream));
             WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * (nitems + 1));
-            k_emit_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, L1.bstart, L1.bounds, L2.bounds, P,
-                                                                    off, items);
+            k_emit_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, lats, 0, P, off, items);
             CF_CHECK(hipGetLastError());
             int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
             CF_CHECK(hipMemsetAsync(counter, 0, sizeof(int), E.stream));
+            const int* d_nitems = off + ncp;
             const int threads = 256;
             // histogram replication: as many copies (power of 2, <= 32) as fit a 40 KiB budget
             const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
@@ -731,7 +768,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
             const long needed = (nitems + 3) / 4;
             if (blocks > needed) blocks = needed > 0 ? needed : 1;
             const void* kfn = nullptr;
-#define CF_K(M, F) if (P.mode == M && P.r2form == F) kfn = (const void*)&k_pairs_tuned<M, F>;
+#define CF_K(M, F) if (P.mode == M && P.r2form == F) kfn = (const void*)&k_pairs_tuned<M, F, false>;
             CF_K(MODE_DD, R2_SUM_XY_Z) CF_K(MODE_DD, R2_FMA_DZ_DY_DX)
             CF_K(MODE_XI, R2_SUM_XY_Z) CF_K(MODE_XI, R2_X_SUM_YZ) CF_K(MODE_XI, R2_FMA_DZ_DY_DX)
             CF_K(MODE_RPPI, RP_SUM) CF_K(MODE_RPPI, RP_FMA) CF_K(MODE_WP, RP_SUM) CF_K(MODE_WP, RP_FMA)
@@ -741,7 +778,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
                 fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d\n", P.mode, P.r2form);
                 return EXIT_FAILURE;
             }
-            void* kargs[] = {&P, &L1, &L2, &cps, &items, &nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w};
+            void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w};
             CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
             CF_CHECK(hipGetLastError());
         }
@@ -777,6 +814,211 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     st.grid_ms = (t_grid - t_start) * 1e3;
     st.total_ms = (now_s() - t_start) * 1e3;
     st.engine = E.engine_kind;
+    E.last = st;
+    return EXIT_SUCCESS;
+}
+
+
+// ----------------------------------------------------------------- batches
+// DD(r) auto-correlations of many independent catalogs in one device work list
+// and one persistent pair-kernel launch. Catalog k gets exactly the results that
+// countpairs() gives for it with a copy of `options`.
+static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, void** W, const char* binfile,
+                     struct config_options* options, struct extra_options* extra, std::vector<CallOut>& outs)
+{
+    Engine& E = eng();
+    const double t_start = now_s();
+    const size_t esize = options->float_type;
+    const int need_w = extra->weight_method != NONE;
+    if (need_w && extra->weight_method != PAIR_PRODUCT) {
+        fprintf(stderr, "Error: unknown weight method %d\n", (int)extra->weight_method);
+        return EXIT_FAILURE;
+    }
+    std::vector<double> rupp;
+    double rmin = 0, rmax = 0;
+    int nbin = 0;
+    if (setup_bins(binfile, &rmin, &rmax, &nbin, rupp) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (!(rmin >= 0.0 && rmax > 0.0 && rmin < rmax && nbin > 0)) {
+        fprintf(stderr, "Error: Could not setup with R bins correctly. (rmin = %lf, rmax = %lf, with nbins = %d).\n",
+                rmin, rmax, nbin);
+        return EXIT_FAILURE;
+    }
+    std::vector<double> rupp_sqr(nbin);
+    for (int i = 0; i < nbin; i++) rupp_sqr[i] = rupp[i] * rupp[i];
+    int sorted = 1;
+    for (int i = 1; i + 1 < nbin; i++)
+        if (!(rupp_sqr[i] >= rupp_sqr[i - 1])) sorted = 0;
+    const int nslots = nbin;
+
+    std::vector<DevLattice> hl(ncat);
+    std::vector<std::pair<CellPairRec*, int>> cpl(ncat);
+    double wrap0[3] = {0, 0, 0}, coord_scale = 0, cs_max = 0, tested_all = 0;
+    int rmaxr = 1, ncp_all = 0;
+    for (int k = 0; k < ncat; k++) {
+        struct config_options oc = *options;
+        const std::string tag = "b" + std::to_string(k);
+        if (need_w && W[k] == nullptr) {
+            fprintf(stderr, "Error: catalog %d has no weights\n", k);
+            return EXIT_FAILURE;
+        }
+        const void *x = to_device(tag + "x", X[k], ND[k], esize), *y = to_device(tag + "y", Y[k], ND[k], esize),
+                   *z = to_device(tag + "z", Z[k], ND[k], esize);
+        const void* w = need_w ? to_device(tag + "w", W[k], ND[k], esize) : nullptr;
+        double ext[6], wrap[3];
+        int per[3];
+        device_minmax(x, y, z, ND[k], esize, ext);
+        if (oc.periodic && oc.boxsize == BOXSIZE_NOTGIVEN) {
+            fprintf(stderr, "boxsize must be specified with periodic wrap.\n");
+            return EXIT_FAILURE;
+        }
+        const double by = oc.boxsize_y == BOXSIZE_NOTGIVEN ? oc.boxsize : oc.boxsize_y;
+        const double bz = oc.boxsize_z == BOXSIZE_NOTGIVEN ? oc.boxsize : oc.boxsize_z;
+        per[0] = oc.periodic && oc.boxsize_x >= 0;
+        per[1] = oc.periodic && by >= 0;
+        per[2] = oc.periodic && bz >= 0;
+        wrap[0] = per[0] ? (oc.boxsize_x > 0 ? oc.boxsize_x : (ext[3] - ext[0])) : 0.;
+        wrap[1] = per[1] ? (by > 0 ? by : (ext[4] - ext[1])) : 0.;
+        wrap[2] = per[2] ? (bz > 0 ? bz : (ext[5] - ext[2])) : 0.;
+        if (k == 0) {
+            memcpy(wrap0, wrap, sizeof wrap0);
+        } else if (wrap[0] != wrap0[0] || wrap[1] != wrap0[1] || wrap[2] != wrap0[2]) {
+            fprintf(stderr, "Error: the catalogs of a batch must share their periodic wraps\n");
+            return EXIT_FAILURE;
+        }
+        int refine[3];
+        GridSpec g;
+        if (choose_grid(MODE_DD, ND[k], rmax, rmax, 0.0, ext, wrap, &oc, g, refine) != EXIT_SUCCESS)
+            return EXIT_FAILURE;
+        if (gridlink_device(tag, g, ND[k], x, y, z, w, esize, hl[k]) != EXIT_SUCCESS) return EXIT_FAILURE;
+        CellPairRec* cps = nullptr;
+        int ncp = 0;
+        double tested = 0;
+        if (cellpairs_device(hl[k], hl[k], refine, per, wrap, rmax, -1, -1, oc.enable_min_sep_opt, 1, &cps, &ncp,
+                             &tested, k) != EXIT_SUCCESS)
+            return EXIT_FAILURE;
+        CellPairRec* keep = (CellPairRec*)E.arena.get(tag + "cp", sizeof(CellPairRec) * (ncp + 1));
+        if (ncp > 0)
+            CF_CHECK(hipMemcpyAsync(keep, cps, sizeof(CellPairRec) * ncp, hipMemcpyDeviceToDevice, E.stream));
+        cpl[k] = {keep, ncp};
+        ncp_all += ncp;
+        tested_all += tested;
+        for (int d = 0; d < 6; d++) coord_scale = fmax(coord_scale, fabs(ext[d]));
+        cs_max = fmax(cs_max, fmax(g.xbs, fmax(g.ybs, g.zbs)));
+        for (int d = 0; d < 3; d++) rmaxr = refine[d] > rmaxr ? refine[d] : rmaxr;
+    }
+    coord_scale = 6 * coord_scale + wrap0[0] + wrap0[1] + wrap0[2] + rmax;
+    const double t_grid = now_s();
+
+    // concatenated work list
+    CellPairRec* cps = (CellPairRec*)E.arena.get("bcp_all", sizeof(CellPairRec) * (ncp_all + 1));
+    {
+        int o = 0;
+        for (int k = 0; k < ncat; k++) {
+            if (cpl[k].second > 0)
+                CF_CHECK(hipMemcpyAsync(cps + o, cpl[k].first, sizeof(CellPairRec) * cpl[k].second,
+                                        hipMemcpyDeviceToDevice, E.stream));
+            o += cpl[k].second;
+        }
+    }
+    DevLattice* lats = (DevLattice*)E.arena.get("blats", sizeof(DevLattice) * ncat);
+    CF_CHECK(hipMemcpyAsync(lats, hl.data(), sizeof(DevLattice) * ncat, hipMemcpyHostToDevice, E.stream));
+
+    PairParams P;
+    std::vector<unsigned short> lut;
+    build_params(MODE_DD, resolve_isa(options->instruction_set), nbin, 0, nslots, rupp_sqr, sorted, rmax, rmax, 0,
+                 0, options->need_avg_sep, need_w, 1, wrap0, coord_scale, cs_max, rmaxr, P, lut);
+
+    const size_t tot = (size_t)ncat * nslots;
+    unsigned long long* g_np = (unsigned long long*)E.arena.get("bh_np", sizeof(uint64_t) * tot);
+    double* g_avg = (double*)E.arena.get("bh_avg", sizeof(double) * tot);
+    double* g_w = (double*)E.arena.get("bh_w", sizeof(double) * tot);
+    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * nbin);
+    unsigned short* d_lut = (unsigned short*)E.arena.get("lut", sizeof(unsigned short) * lut.size());
+    CF_CHECK(hipMemsetAsync(g_np, 0, sizeof(uint64_t) * tot, E.stream));
+    CF_CHECK(hipMemsetAsync(g_avg, 0, sizeof(double) * tot, E.stream));
+    CF_CHECK(hipMemsetAsync(g_w, 0, sizeof(double) * tot, E.stream));
+    CF_CHECK(hipMemcpyAsync(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice, E.stream));
+    CF_CHECK(hipMemcpyAsync(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice, E.stream));
+
+    int nitems = 0;
+    CF_CHECK(hipEventRecord(E.ev0, E.stream));
+    if (ncp_all > 0) {
+        int* nch = (int*)E.arena.get("wi_n", sizeof(int) * (ncp_all + 1));
+        int* off = (int*)E.arena.get("wi_off", sizeof(int) * (ncp_all + 1));
+        k_count_chunks<<<(ncp_all + 255) / 256, 256, 0, E.stream>>>(ncp_all, cps, lats, 1, nch);
+        CF_CHECK(hipGetLastError());
+        device_scan(nch, off, ncp_all, "wi_scan");
+        CF_CHECK(hipMemcpyAsync(&nitems, off + ncp_all, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * (nitems + 1));
+        k_emit_chunks<<<(ncp_all + 255) / 256, 256, 0, E.stream>>>(ncp_all, cps, lats, 1, P, off, items);
+        CF_CHECK(hipGetLastError());
+        int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
+        CF_CHECK(hipMemsetAsync(counter, 0, sizeof(int), E.stream));
+        const int* d_nitems = off + ncp_all;
+        const int threads = 256, nwaves = threads / 64;
+        const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
+        int R = 1;
+        while (R < 32 && (size_t)nwaves * nslots * per_slot * (2 * R) <= 40 * 1024) R *= 2;
+        P.hist_copies = R;
+        const size_t lds = (size_t)nwaves * nslots * per_slot * R + (size_t)nbin * 8 +
+                           (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueue * 5;
+        if (lds > 64 * 1024) {
+            fprintf(stderr, "Error: LDS requirement %zu B of the batched pair kernel exceeds 64 KiB\n", lds);
+            return EXIT_FAILURE;
+        }
+        int ncu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, E.device) == hipSuccess) ncu = prop.multiProcessorCount;
+        int per_cu = (int)((160 * 1024) / (lds + 1024));
+        per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+        long blocks = (long)ncu * per_cu;
+        const long needed = (nitems + 3) / 4;
+        if (blocks > needed) blocks = needed > 0 ? needed : 1;
+        const void* kfn = P.r2form == R2_FMA_DZ_DY_DX ? (const void*)&k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true>
+                                                     : (const void*)&k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true>;
+        void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w};
+        CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
+    }
+    CF_CHECK(hipEventRecord(E.ev1, E.stream));
+    std::vector<uint64_t> np(tot);
+    std::vector<double> avg(tot), wv(tot);
+    CF_CHECK(hipMemcpyAsync(np.data(), g_np, sizeof(uint64_t) * tot, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(avg.data(), g_avg, sizeof(double) * tot, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(wv.data(), g_w, sizeof(double) * tot, hipMemcpyDeviceToHost, E.stream));
+    std::vector<double> wsums(2 * (size_t)ncat, 0.0);
+    if (need_w) {
+        double* ws = (double*)E.arena.get("bwsums", sizeof(double) * 2 * ncat);
+        CF_CHECK(hipMemsetAsync(ws, 0, sizeof(double) * 2 * ncat, E.stream));
+        for (int k = 0; k < ncat; k++)
+            if (ND[k] > 0) k_weight_sums<<<64, 256, 0, E.stream>>>(ND[k], hl[k].w, ws + 2 * k);
+        CF_CHECK(hipGetLastError());
+        CF_CHECK(hipMemcpyAsync(wsums.data(), ws, sizeof(double) * 2 * ncat, hipMemcpyDeviceToHost, E.stream));
+    }
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    float kms = 0;
+    CF_CHECK(hipEventElapsedTime(&kms, E.ev0, E.ev1));
+    outs.assign(ncat, CallOut());
+    for (int k = 0; k < ncat; k++) {
+        CallOut& R = outs[k];
+        R.rupp = rupp;
+        R.nbin = nbin;
+        R.need_w = need_w;
+        R.np.assign(np.begin() + (size_t)k * nslots, np.begin() + (size_t)(k + 1) * nslots);
+        R.avg.assign(avg.begin() + (size_t)k * nslots, avg.begin() + (size_t)(k + 1) * nslots);
+        R.wavg.assign(wv.begin() + (size_t)k * nslots, wv.begin() + (size_t)(k + 1) * nslots);
+        R.wsum = wsums[2 * k];
+        R.wsq = wsums[2 * k + 1];
+    }
+    cf_amd_stats st;
+    memset(&st, 0, sizeof st);
+    st.num_cell_pairs = ncp_all;
+    st.num_work_items = nitems;
+    st.tested_pairs = tested_all;
+    st.kernel_ms = kms;
+    st.grid_ms = (t_grid - t_start) * 1e3;
+    st.total_ms = (now_s() - t_start) * 1e3;
+    st.engine = CF_AMD_ENGINE_TUNED;
     E.last = st;
     return EXIT_SUCCESS;
 }
@@ -1184,6 +1426,50 @@ int cf_amd_last_stats(cf_amd_stats* out)
     if (!out) return EXIT_FAILURE;
     return guarded([&]() -> int {
         *out = eng().last;
+        return EXIT_SUCCESS;
+    });
+}
+
+
+int cf_amd_countpairs_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, void** W, const int numthreads,
+                            const char* binfile, results_countpairs* results, struct config_options* options,
+                            struct extra_options* extra)
+{
+    (void)numthreads;
+    if (ncat <= 0 || !ND || !X || !Y || !Z || !results) return EXIT_FAILURE;
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        const double t0 = now_s();
+        fix_maxcells(options);
+        fix_refine(options);
+        options->sort_on_z = 1;
+        std::vector<CallOut> outs;
+        if (run_batch(ncat, ND, X, Y, Z, W, binfile, options, extra, outs) != EXIT_SUCCESS) return EXIT_FAILURE;
+        for (int k = 0; k < ncat; k++) {
+            CallOut& R = outs[k];
+            finalize(R, 1, ND[k], options->need_avg_sep, 1);
+            const int nbin = R.nbin;
+            results_countpairs* r = &results[k];
+            r->nbin = nbin;
+            r->npairs = (uint64_t*)malloc(sizeof(uint64_t) * nbin);
+            r->rupp = (double*)malloc(sizeof(double) * nbin);
+            r->rpavg = (double*)calloc(nbin, sizeof(double));
+            r->weightavg = (double*)calloc(nbin, sizeof(double));
+            if (!r->npairs || !r->rupp || !r->rpavg || !r->weightavg) {
+                for (int q = 0; q <= k; q++) free_results(&results[q]);
+                return EXIT_FAILURE;
+            }
+            for (int i = 0; i < nbin; i++) {
+                r->npairs[i] = R.np[i];
+                r->rupp[i] = R.rupp[i];
+                r->rpavg[i] = options->need_avg_sep ? R.avg[i] : 0.0;
+                r->weightavg[i] = R.need_w ? R.wavg[i] : 0.0;
+            }
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
         return EXIT_SUCCESS;
     });
 }

@@ -336,13 +336,14 @@ __global__ void k_cellpair_flag(NgbArgs a, int64_t ncells, int nngb, const int* 
 __global__ void k_cellpair_emit(NgbArgs a, int64_t ncells, int nngb, const int* __restrict__ bs1,
                                 const int* __restrict__ bs2, const double* __restrict__ bd1,
                                 const double* __restrict__ bd2, const int* __restrict__ pos,
-                                CellPairRec* __restrict__ out, double* __restrict__ tested)
+                                CellPairRec* __restrict__ out, double* __restrict__ tested, int cat)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     double my = 0;
     if (t < ncells * nngb) {
         CellPairRec r;
         if (neighbour_added(a, t / nngb, (int)(t % nngb), bs1, bs2, bd1, bd2, &r)) {
+            r.pad = cat;
             out[pos[t]] = r;
             const double n1 = (double)cell_count(bs1, r.c1), n2 = (double)cell_count(bs2, r.c2);
             my = r.same ? n1 * (n1 - 1) * 0.5 : n1 * n2;
@@ -686,14 +687,14 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
 
 template <int MODE, int R2F>
 __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const DevLattice& L1,
-                                            const DevLattice& L2, int i0, const int* qj, const unsigned char* ql,
-                                            int head, int ndrain, KernelLds& S)
+                                            const DevLattice& L2, int i0, int istride, const int* qj,
+                                            const unsigned char* ql, int head, int ndrain, KernelLds& S)
 {
     const int lane = lane_id();
     if (lane < ndrain) {
         const int slotq = (head + lane) & (kQueue - 1);
         const int j = qj[slotq];
-        const int i = i0 + ql[slotq];
+        const int i = i0 + ql[slotq] * istride;
         const double4 pi = L1.d4[i];
         const double4 pj = L2.d4[j];
         exact_pair<MODE, R2F>(P, cp, L1, i, 1, pi.x + cp.wx * P.xwrap, pi.y + cp.wy * P.ywrap,
@@ -701,11 +702,143 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
     }
 }
 
+// One work item of the tuned kernel (see the comment above kQueue).
 template <int MODE, int R2F>
-__global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1, DevLattice L2,
+__device__ __forceinline__ void process_item(const PairParams& P, const DevLattice& L1, const DevLattice& L2,
+                                             const CellPairRec& cp, const WorkItem& it, KernelLds& S, int* qj,
+                                             unsigned char* ql)
+{
+    const int lane = lane_id();
+    const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
+    uniform_f4_ptr f4j = (uniform_f4_ptr)L2.f4;
+    uniform_d4_ptr d4j = (uniform_d4_ptr)L2.d4;
+    const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
+    const int flags = __builtin_amdgcn_readfirstlane(it.flags);
+    const int dense = P.dense_mode == 1 ? 1 : (P.dense_mode == 2 ? 0 : (flags & kItemDense));
+    // dense items take strided primaries (whole-cell sample per wave), which
+    // spreads the lanes' pairs over more histogram slots
+    const int stride = flags >> kStrideShift;
+    const int i = i0 + lane * (stride > 0 ? stride : 1);
+    const int ivalid = i < i1;
+    const int c2 = cp.c2;
+    double xpos = 0, ypos = 0, zpos = 0, wi = 0;
+    if (ivalid) {
+        const double4 pi = L1.d4[i];
+        xpos = pi.x + cp.wx * P.xwrap;
+        ypos = pi.y + cp.wy * P.ywrap;
+        zpos = pi.z + cp.wz * P.zwrap;
+        wi = pi.w;
+    }
+    // secondary z-slabs this chunk can reach
+    const int iz2 = c2 % L2.nz;
+    const double zlo = wave_min_d(ivalid ? zpos : DBL_MAX) - P.zreach - P.slab_margin;
+    const double zhi = wave_max_d(ivalid ? zpos : -DBL_MAX) + P.zreach + P.slab_margin;
+    int jlo, jhi;
+    {
+        const int64_t cb = (int64_t)c2 * kSlabs;
+        double flo = floor(((zlo - L2.zmin) * L2.zinv - (double)iz2) * kSlabs);
+        double fhi = floor(((zhi - L2.zmin) * L2.zinv - (double)iz2) * kSlabs);
+        if (L2.zinv == 0.0) { flo = 0.0; fhi = kSlabs - 1; }
+        if (fhi < 0.0 || flo > (double)(kSlabs - 1)) {
+            jlo = jhi = 0;
+        } else {
+            const int slo = flo < 0.0 ? 0 : (int)flo;
+            const int shi = fhi > (double)(kSlabs - 1) ? kSlabs - 1 : (int)fhi;
+            jlo = L2.bstart[cb + slo];
+            jhi = L2.bstart[cb + shi + 1];
+        }
+    }
+    if (cp.same && jlo < i0 + 1) jlo = i0 + 1;
+    jlo = __builtin_amdgcn_readfirstlane(jlo);
+    jhi = __builtin_amdgcn_readfirstlane(jhi);
+
+    if (dense) {
+        const int same = cp.same;
+        for (int j = jlo; j < jhi; j++) {
+            const double xj = d4j[j].x, yj = d4j[j].y, zj = d4j[j].z, wj = d4j[j].w;
+            exact_pair<MODE, R2F>(P, cp, L1, i, ivalid && (!same || j > i), xpos, ypos, zpos, wi, xj, yj, zj, wj,
+                                  S);
+        }
+        return;
+    }
+
+    const int ixy2 = c2 / L2.nz, iy2 = ixy2 % L2.ny, ix2 = ixy2 / L2.ny;
+    const double ox = L2.xmin + ix2 * L2.xbs, oy = L2.ymin + iy2 * L2.ybs, oz = L2.zmin + iz2 * L2.zbs;
+    const float xr = (float)(xpos - ox), yr = (float)(ypos - oy), zr_i = (float)(zpos - oz);
+    const int istride = stride > 0 ? stride : 1;
+    int head = 0, tail = 0, count = 0;
+    for (int j = jlo; j < jhi; j++) {
+        const float pjx = f4j[j].x, pjy = f4j[j].y, pjz = f4j[j].z;
+        const float dx = pjx - xr, dy = pjy - yr, dz = pjz - zr_i;
+        float q;
+        int pass;
+        if (MODE == MODE_RPPI || MODE == MODE_WP) {
+            q = fmaf(dy, dy, dx * dx);
+            pass = ivalid && q < hi2 && q >= lo2 && fabsf(dz) < zr;
+        } else {
+            q = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            pass = ivalid && q < hi2 && q >= lo2;
+        }
+        if (cp.same) pass = pass && (j > i);
+        const unsigned long long m = __ballot(pass);
+        if (m) {
+            if (pass) {
+                const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+                const int pos = (tail + before) & (kQueue - 1);
+                qj[pos] = j;
+                ql[pos] = (unsigned char)lane;
+            }
+            const int added = __popcll(m);
+            tail += added;
+            count += added;
+            if (count >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qj, ql, head, 64, S);
+                __builtin_amdgcn_wave_barrier();
+                head += 64;
+                count -= 64;
+            }
+        }
+    }
+    if (count > 0) {
+        __builtin_amdgcn_wave_barrier();
+        drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qj, ql, head, count, S);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Adds a (replicated) LDS histogram region to the global histogram at `base`
+// and clears it; `t0`/`tstep` split the slots among the calling threads.
+__device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, int R, int64_t base, int t0,
+                                             int tstep, unsigned long long* g_np, double* g_avg, double* g_w)
+{
+    for (int sl = t0; sl < P.nslots; sl += tstep) {
+        unsigned long long c = 0;
+        double sa = 0, sw = 0;
+        for (int r = 0; r < R; r++) {
+            c += S.np[sl * R + r];
+            S.np[sl * R + r] = 0;
+            if (P.need_avg) { sa += S.avg[sl * R + r]; S.avg[sl * R + r] = 0; }
+            if (P.need_w) { sw += S.w[sl * R + r]; S.w[sl * R + r] = 0; }
+        }
+        if (c) {
+            atomicAdd(&g_np[base + sl], c);
+            if (P.need_avg) atomicAdd(&g_avg[base + sl], sa);
+            if (P.need_w) atomicAdd(&g_w[base + sl], sw);
+        }
+    }
+}
+
+// The tuned pair kernel. lats[0], lats[1]: primary / secondary particle sets;
+// in a batch (BATCH = true) every cell pair carries its catalog index k in
+// `pad`, both sets are lats[k], each wave keeps its own LDS histogram for the
+// catalog it is working on, and the histogram of catalog k lives at
+// g_*[k*nslots ...].
+template <int MODE, int R2F, bool BATCH>
+__global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
-                                                     const WorkItem* __restrict__ items, int nitems,
-                                                     int* __restrict__ counter,
+                                                     const WorkItem* __restrict__ items,
+                                                     const int* __restrict__ nitems_p, int* __restrict__ counter,
                                                      const double* __restrict__ rupp_sqr,
                                                      const unsigned short* __restrict__ lut_g,
                                                      unsigned long long* __restrict__ g_np,
@@ -714,32 +847,37 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
     extern __shared__ unsigned long long smem_u64[];
     KernelLds S;
     const int R = P.hist_copies;
-    const int nrep = P.nslots * R;
-    S.np = smem_u64;
-    S.avg = reinterpret_cast<double*>(smem_u64 + nrep);
-    S.w = S.avg + (P.need_avg ? nrep : 0);
-    double* e = S.w + (P.need_w ? nrep : 0);
+    const int nwaves = blockDim.x >> 6;
+    const int nrep = P.nslots * R;                  // one histogram region
+    const int nreg = BATCH ? nrep * nwaves : nrep;  // all regions of the workgroup
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    unsigned long long* np0 = smem_u64;
+    double* avg0 = reinterpret_cast<double*>(smem_u64 + nreg);
+    double* w0 = avg0 + (P.need_avg ? nreg : 0);
+    double* e = w0 + (P.need_w ? nreg : 0);
+    const int roff = BATCH ? wave * nrep : 0;
+    S.np = np0 + roff;
+    S.avg = avg0 + roff;
+    S.w = w0 + roff;
     S.rmask = R - 1;
-    S.lane_copy = lane_id() & (R - 1);
+    S.lane_copy = lane & (R - 1);
     unsigned short* lut = reinterpret_cast<unsigned short*>(e + P.nbin);
     int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
     S.e = e;
     S.lut = lut;
-    const int wave = threadIdx.x >> 6, lane = lane_id();
     int* qj = qbase + wave * kQueue;
-    unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + (blockDim.x >> 6) * kQueue) + wave * kQueue;
+    unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + nwaves * kQueue) + wave * kQueue;
     for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
     for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
-    for (int s = threadIdx.x; s < nrep; s += blockDim.x) {
-        S.np[s] = 0;
-        if (P.need_avg) S.avg[s] = 0;
-        if (P.need_w) S.w[s] = 0;
+    for (int s = threadIdx.x; s < nreg; s += blockDim.x) {
+        np0[s] = 0;
+        if (P.need_avg) avg0[s] = 0;
+        if (P.need_w) w0[s] = 0;
     }
     __syncthreads();
-    const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
-    uniform_f4_ptr f4j = (uniform_f4_ptr)L2.f4;
-    uniform_d4_ptr d4j = (uniform_d4_ptr)L2.d4;
-
+    const int nitems = *nitems_p;
+    DevLattice L1 = lats[0], L2 = lats[BATCH ? 0 : 1];
+    int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(counter, 1);
@@ -747,152 +885,92 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, DevLattice L1
         if (item >= nitems) break;
         const WorkItem it = items[item];
         const CellPairRec cp = cps[__builtin_amdgcn_readfirstlane(it.cp)];
-        const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
-        const int dense = P.dense_mode == 1 ? 1 : (P.dense_mode == 2 ? 0 : (__builtin_amdgcn_readfirstlane(it.flags) & kItemDense));
-        const int i = i0 + lane;
-        const int ivalid = i < i1;
-        const int c2 = cp.c2;
-        double xpos = 0, ypos = 0, zpos = 0, wi = 0;
-        if (ivalid) {
-            const double4 pi = L1.d4[i];
-            xpos = pi.x + cp.wx * P.xwrap;
-            ypos = pi.y + cp.wy * P.ywrap;
-            zpos = pi.z + cp.wz * P.zwrap;
-            wi = pi.w;
-        }
-        // secondary z-slabs this chunk can reach
-        const int iz2 = c2 % L2.nz;
-        const double zlo = wave_min_d(ivalid ? zpos : DBL_MAX) - P.zreach - P.slab_margin;
-        const double zhi = wave_max_d(ivalid ? zpos : -DBL_MAX) + P.zreach + P.slab_margin;
-        int jlo, jhi;
-        {
-            const int64_t cb = (int64_t)c2 * kSlabs;
-            double flo = floor(((zlo - L2.zmin) * L2.zinv - (double)iz2) * kSlabs);
-            double fhi = floor(((zhi - L2.zmin) * L2.zinv - (double)iz2) * kSlabs);
-            if (L2.zinv == 0.0) { flo = 0.0; fhi = kSlabs - 1; }
-            if (fhi < 0.0 || flo > (double)(kSlabs - 1)) {
-                jlo = jhi = 0;
-            } else {
-                const int slo = flo < 0.0 ? 0 : (int)flo;
-                const int shi = fhi > (double)(kSlabs - 1) ? kSlabs - 1 : (int)fhi;
-                jlo = L2.bstart[cb + slo];
-                jhi = L2.bstart[cb + shi + 1];
-            }
-        }
-        if (cp.same && jlo < i0 + 1) jlo = i0 + 1;
-        jlo = __builtin_amdgcn_readfirstlane(jlo);
-        jhi = __builtin_amdgcn_readfirstlane(jhi);
-
-        if (dense) {
-            const int same = cp.same;
-            for (int j = jlo; j < jhi; j++) {
-                const double xj = d4j[j].x, yj = d4j[j].y, zj = d4j[j].z, wj = d4j[j].w;
-                exact_pair<MODE, R2F>(P, cp, L1, i, ivalid && (!same || j > i), xpos, ypos, zpos, wi, xj, yj, zj,
-                                      wj, S);
-            }
-            continue;
-        }
-
-        const int iz2b = c2 % L2.nz, ixy2 = c2 / L2.nz, iy2 = ixy2 % L2.ny, ix2 = ixy2 / L2.ny;
-        const double ox = L2.xmin + ix2 * L2.xbs, oy = L2.ymin + iy2 * L2.ybs, oz = L2.zmin + iz2b * L2.zbs;
-        const float xr = (float)(xpos - ox), yr = (float)(ypos - oy), zr_i = (float)(zpos - oz);
-        int head = 0, tail = 0, count = 0;
-        for (int j = jlo; j < jhi; j++) {
-            const float pjx = f4j[j].x, pjy = f4j[j].y, pjz = f4j[j].z;
-            const float dx = pjx - xr, dy = pjy - yr, dz = pjz - zr_i;
-            float q;
-            int pass;
-            if (MODE == MODE_RPPI || MODE == MODE_WP) {
-                q = fmaf(dy, dy, dx * dx);
-                pass = ivalid && q < hi2 && q >= lo2 && fabsf(dz) < zr;
-            } else {
-                q = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                pass = ivalid && q < hi2 && q >= lo2;
-            }
-            if (cp.same) pass = pass && (j > i);
-            const unsigned long long m = __ballot(pass);
-            if (m) {
-                if (pass) {
-                    const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-                    const int pos = (tail + before) & (kQueue - 1);
-                    qj[pos] = j;
-                    ql[pos] = (unsigned char)lane;
-                }
-                const int added = __popcll(m);
-                tail += added;
-                count += added;
-                if (count >= 64) {
+        if (BATCH) {
+            const int k = __builtin_amdgcn_readfirstlane(cp.pad);
+            if (k != cur) {
+                if (cur >= 0) {
                     __builtin_amdgcn_wave_barrier();
-                    drain_queue<MODE, R2F>(P, cp, L1, L2, i0, qj, ql, head, 64, S);
+                    flush_region(P, S, R, (int64_t)cur * P.nslots, lane, 64, g_np, g_avg, g_w);
                     __builtin_amdgcn_wave_barrier();
-                    head += 64;
-                    count -= 64;
                 }
+                cur = k;
+                L1 = lats[k];
+                L2 = L1;
             }
         }
-        if (count > 0) {
+        process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
+    }
+    if (BATCH) {
+        if (cur >= 0) {
             __builtin_amdgcn_wave_barrier();
-            drain_queue<MODE, R2F>(P, cp, L1, L2, i0, qj, ql, head, count, S);
-            __builtin_amdgcn_wave_barrier();
+            flush_region(P, S, R, (int64_t)cur * P.nslots, lane, 64, g_np, g_avg, g_w);
         }
+        return;
     }
     __syncthreads();
-    for (int sl = threadIdx.x; sl < P.nslots; sl += blockDim.x) {
-        unsigned long long c = 0;
-        double sa = 0, sw = 0;
-        for (int r = 0; r < R; r++) {
-            c += S.np[sl * R + r];
-            if (P.need_avg) sa += S.avg[sl * R + r];
-            if (P.need_w) sw += S.w[sl * R + r];
-        }
-        if (c) {
-            atomicAdd(&g_np[sl], c);
-            if (P.need_avg) atomicAdd(&g_avg[sl], sa);
-            if (P.need_w) atomicAdd(&g_w[sl], sw);
-        }
-    }
+    flush_region(P, S, R, 0, threadIdx.x, blockDim.x, g_np, g_avg, g_w);
 }
 
-template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_XI, R2_SUM_XY_Z>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_XI, R2_X_SUM_YZ>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_XI, R2_FMA_DZ_DY_DX>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_RPPI, RP_SUM>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_RPPI, RP_FMA>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_WP, RP_SUM>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_WP, RP_FMA>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_SMU, R2_SUM_XY_Z>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
-template __global__ void k_pairs_tuned<MODE_SMU, R2_FMA_DX_DY_DZ>(PairParams, DevLattice, DevLattice, const CellPairRec*,
-    const WorkItem*, int, int*, const double*, const unsigned short*, unsigned long long*, double*, double*);
+template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_XI, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_XI, R2_X_SUM_YZ, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_XI, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_RPPI, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_RPPI, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_WP, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_WP, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_SMU, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_SMU, R2_FMA_DX_DY_DZ, false>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
+template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true>(PairParams, const DevLattice*, const CellPairRec*,
+    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
+    double*);
 
 // Work-item construction: chunks of 64 primary particles per cell pair.
-__global__ void k_count_chunks(int ncp, const CellPairRec* __restrict__ cps, const int* __restrict__ bs1,
-                               int* __restrict__ nchunks)
+__global__ void k_count_chunks(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
+                               int batch, int* __restrict__ nchunks)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncp) return;
+    const int* bs1 = lats[batch ? cps[t].pad : 0].bstart;
     const int n = (int)cell_count(bs1, cps[t].c1);
     nchunks[t] = (n + 63) / 64;
 }
 
-__global__ void k_emit_chunks(int ncp, const CellPairRec* __restrict__ cps, const int* __restrict__ bs1,
-                              const double* __restrict__ bd1, const double* __restrict__ bd2, PairParams P,
-                              const int* __restrict__ off, WorkItem* __restrict__ items)
+__global__ void k_emit_chunks(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
+                              int batch, PairParams P, const int* __restrict__ off, WorkItem* __restrict__ items)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncp) return;
     const CellPairRec cp = cps[t];
+    const int* bs1 = lats[batch ? cp.pad : 0].bstart;
+    const double* bd1 = lats[batch ? cp.pad : 0].bounds;
+    const double* bd2 = lats[batch ? cp.pad : 1].bounds;
     const int c1 = cp.c1;
     // Largest separation between the two cells' bounding boxes (primary box
     // shifted by the wrap): when it is inside the range, every pair is.
@@ -909,12 +987,19 @@ __global__ void k_emit_chunks(int ncp, const CellPairRec* __restrict__ cps, cons
         dense = mx * mx + my * my + mz * mz < 0.98 * P.sqr_rmax;
     const int s = bs1[(int64_t)c1 * kSlabs], e = bs1[(int64_t)(c1 + 1) * kSlabs];
     int o = off[t];
-    for (int i = s; i < e; i += 64) {
+    const int nch = (e - s + 63) / 64;
+    for (int c = 0; c < nch; c++) {
         WorkItem w;
         w.cp = t;
-        w.i0 = i;
-        w.i1 = i + 64 < e ? i + 64 : e;
-        w.flags = dense ? kItemDense : 0;
+        if (dense && P.dense_stride && !cp.same) {
+            w.i0 = s + c;
+            w.i1 = e;
+            w.flags = kItemDense | (nch << kStrideShift);
+        } else {
+            w.i0 = s + 64 * c;
+            w.i1 = w.i0 + 64 < e ? w.i0 + 64 : e;
+            w.flags = dense ? kItemDense : 0;
+        }
         items[o++] = w;
     }
 }

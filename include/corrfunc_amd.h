@@ -15,6 +15,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "corrfunc_theory.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -48,6 +50,18 @@ int cf_amd_get_engine(void);
 int cf_amd_last_stats(cf_amd_stats *out);
 /* Library identification string. */
 const char *cf_amd_version(void);
+
+/* DD(r) auto-correlations of `ncat` independent catalogs (mocks, jackknife or
+ * bootstrap resamples) in one call: every catalog is gridded on the GPU, their
+ * cell pairs form one work list and one persistent kernel launch counts them all.
+ * results[k] equals what countpairs(ND[k], X[k], Y[k], Z[k], 0, ..., autocorr=1,
+ * binfile, &results[k], &copy_of_options, extra) returns; release each with
+ * free_results. W[k] are the weights of catalog k (used when
+ * extra->weight_method = PAIR_PRODUCT; W may be NULL otherwise). Pointers may be
+ * host or device memory. All catalogs must share the periodic wraps. */
+int cf_amd_countpairs_batch(int ncat, const int64_t *ND, void **X, void **Y, void **Z, void **W,
+                            const int numthreads, const char *binfile, results_countpairs *results,
+                            struct config_options *options, struct extra_options *extra);
 
 /* Device memory on the engine's GPU, from the engine's own HIP runtime, for
  * callers that keep catalogs resident in HBM across calls. */

@@ -98,3 +98,27 @@ def test_device_resident_inputs(engine, oracle, eng, dtype):
                autocorr=1, nthreads=2, binfile=bins, X1=X[0], Y1=X[1], Z1=X[2], weights1=W,
                weight_type="pair_product")
     assert_same("countpairs", got, want)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_batch_matches_single_calls_and_oracle(engine, oracle, weighted, dtype):
+    """cf_amd_countpairs_batch: each catalog equals countpairs() on it and the oracle."""
+    _select("tuned")
+    L = 120.0
+    rng = np.random.default_rng(77)
+    sizes = [int(v) for v in np.exp(rng.uniform(np.log(300), np.log(20000), 12))]
+    cats = [uniform(n, L, 1000 + k, dtype) for k, n in enumerate(sizes)]
+    cats[3] = [np.ascontiguousarray(a) for a in clustered(sizes[3], L, 5, nparent=6, sigma=1.5, dtype=dtype)]
+    ws = [weights(n, 2000 + k, dtype) for k, n in enumerate(sizes)] if weighted else None
+    bins = np.logspace(np.log10(0.5), np.log10(18.0), 12)
+    o = dict(periodic=True, boxsize=L, need_avg_sep=True, isa="fallback")
+    outs = _lib.countpairs_batch([tuple(c) for c in cats], bins, options=_opts(**o), weights=ws)
+    for k, c in enumerate(cats):
+        kw = dict(autocorr=1, nthreads=2, binfile=bins, X1=c[0], Y1=c[1], Z1=c[2])
+        if weighted:
+            kw.update(weights1=ws[k], weight_type="pair_product")
+        single = run(engine, "countpairs", options=_opts(**o), **kw)
+        want = run(oracle, "countpairs", "oracle_", options=_opts(**o), **kw)
+        assert_same("countpairs", outs[k], want, what="catalog %d" % k)
+        assert np.array_equal(outs[k]["npairs"], single["npairs"])
