@@ -334,6 +334,33 @@ def call_mode(lib, mode, prefix="", *, autocorr=1, nthreads=1, binfile=None, X1=
     out = _collect(mode, res)
     out["api_time"] = o.c_api_time
     getattr(lib, prefix + _FREE[mode])(C.byref(res))
+    if o.c_cell_timer:
+        out["cell_timings"] = take_cell_timings(o)
+    return out
+
+
+CELL_TIMINGS_DTYPE = np.dtype([("N1", np.int64), ("N2", np.int64), ("time_in_ns", np.int64),
+                               ("cellidx1", np.int32), ("cellidx2", np.int32), ("tid", np.int32)])
+
+
+def take_cell_timings(o):
+    """Copies options->cell_timings (totncells_timings rows) into a structured
+    array with the fields of the reference's _convert_cell_timer
+    (Corrfunc/theory/wp.py:244-280) and releases it as free_cell_timings does."""
+    n = int(o.totncells_timings)
+    out = np.zeros(max(n, 0), dtype=CELL_TIMINGS_DTYPE)
+    if n > 0 and o.cell_timings:
+        raw_dt = np.dtype({"names": list(CELL_TIMINGS_DTYPE.names),
+                           "formats": [np.int64, np.int64, np.int64, np.int32, np.int32, np.int32],
+                           "offsets": [0, 8, 16, 24, 28, 32], "itemsize": C.sizeof(api_cell_timings)})
+        raw = np.ctypeslib.as_array(C.cast(o.cell_timings, C.POINTER(C.c_uint8)),
+                                    shape=(n * C.sizeof(api_cell_timings),))
+        rows = raw.view(raw_dt)
+        for name in CELL_TIMINGS_DTYPE.names:
+            out[name] = rows[name]
+        C.CDLL(None).free(C.cast(o.cell_timings, C.c_void_p))
+    o.cell_timings = None
+    o.totncells_timings = 0
     return out
 
 

@@ -15,11 +15,22 @@ _lib = None
 class cf_amd_stats(C.Structure):
     _fields_ = [("nmesh", C.c_int * 3), ("refine", C.c_int * 3), ("num_cell_pairs", C.c_int64),
                 ("num_work_items", C.c_int64), ("tested_pairs", C.c_double), ("kernel_ms", C.c_double),
-                ("grid_ms", C.c_double), ("total_ms", C.c_double), ("engine", C.c_int), ("pad", C.c_int)]
+                ("grid_ms", C.c_double), ("total_ms", C.c_double), ("engine", C.c_int), ("sched", C.c_int),
+                ("list_ms", C.c_double)]
 
 
 ENGINE_TUNED = 0
 ENGINE_PLAIN = 1
+
+SCHED_NONE = 0
+SCHED_HEURISTIC = 1
+SCHED_LEARNED = 2
+
+MODEL_FEATURES = 10
+MODEL_PARAMS = 1441
+MODEL_NORM = 22
+
+_FP = C.POINTER(C.c_float)
 
 
 def lib():
@@ -51,6 +62,22 @@ def lib():
                                               C.c_int, C.c_char_p, C.POINTER(_abi.results_countpairs),
                                               C.POINTER(_abi.config_options), C.POINTER(_abi.extra_options)]
         L.cf_amd_countpairs_batch.restype = C.c_int
+        for name, args, res in [
+                ("cf_amd_set_scheduler", [C.c_int], C.c_int),
+                ("cf_amd_get_scheduler", [], C.c_int),
+                ("cf_amd_record_costs", [C.c_int], C.c_int),
+                ("cf_amd_cost_samples", [_FP, _FP, C.c_int64], C.c_int64),
+                ("cf_amd_clear_cost_samples", [], C.c_int),
+                ("cf_amd_model_train", [_FP, _FP, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint, _FP], C.c_int),
+                ("cf_amd_model_get", [_FP, _FP], C.c_int),
+                ("cf_amd_model_set", [_FP, _FP], C.c_int),
+                ("cf_amd_model_reset", [], C.c_int),
+                ("cf_amd_model_trained", [], C.c_int),
+                ("cf_amd_model_predict", [_FP, C.c_int64, _FP], C.c_int),
+                ("cf_amd_model_grad", [_FP, _FP, C.c_int64, _FP, _FP, _FP, _FP], C.c_int)]:
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
         _lib = L
     return _lib
 
@@ -69,7 +96,7 @@ def last_stats():
     lib().cf_amd_last_stats(C.byref(s))
     return dict(nmesh=tuple(s.nmesh), refine=tuple(s.refine), num_cell_pairs=s.num_cell_pairs,
                 num_work_items=s.num_work_items, tested_pairs=s.tested_pairs, kernel_ms=s.kernel_ms,
-                grid_ms=s.grid_ms, total_ms=s.total_ms, engine=s.engine)
+                grid_ms=s.grid_ms, total_ms=s.total_ms, engine=s.engine, sched=s.sched, list_ms=s.list_ms)
 
 
 class DeviceArray:
@@ -94,6 +121,14 @@ class DeviceArray:
         if lib().cf_amd_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1) != 0:
             raise RuntimeError("copy from device failed")
         return out
+
+    def copy_from(self, other):
+        """Device-to-device copy of another DeviceArray of the same size (the
+        engine's own copy kernel)."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch: %d vs %d bytes" % (other.nbytes, self.nbytes))
+        if lib().cf_amd_memcpy(self.ptr, other.ptr, self.nbytes, 2) != 0:
+            raise RuntimeError("device-to-device copy failed")
 
     def free(self):
         if self.ptr:

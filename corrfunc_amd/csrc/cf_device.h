@@ -77,13 +77,29 @@ struct PairParams {
     int pad_i2;
 };
 
-// Work item of the tuned kernel: (cell pair, <= 64 consecutive primary particles).
+// Work item of the tuned kernel: (cell pair, <= 64 primary particles, one part
+// of their secondary window).
 struct WorkItem {
     int cp;
-    int i0, i1;  // contiguous items: primaries [i0, i1); strided: i0 + lane*stride < i1
-    int flags;   // bit 0: dense (every pair of the cell pair inside the range); bits 8..: stride
+    int i0, i1;      // contiguous items: primaries [i0, i1); strided: i0 + lane*stride < i1
+    int flags;       // bit 0: dense (every pair of the cell pair inside the range); bits 8..: stride
+    int part, nparts;  // the item scans part `part` of `nparts` equal slices of the secondary window
 };
 constexpr int kItemDense = 1;
 constexpr int kStrideShift = 8;
+constexpr int kMaxParts = 32;
+
+// Per-cell-pair features of the cost model (computed on the device).
+constexpr int kNumFeatures = 10;
+// f0 log2(1+N1)       f1 log2(1+N2)        f2 log2(chunks of 64 primaries)
+// f3 same cell        f4 dense (bounding boxes entirely in range)
+// f5 min separation^2 / rmax^2              f6 max separation^2 / rmax^2 (<= 8)
+// f7 fraction of the secondary cell's z extent within z reach of the primaries
+// f8 projected (rp, pi) mode                f9 log2(1 + N2 * f7)
+
+// Cost-ordered work lists: cell pairs are bucketed by log2 cost, descending.
+constexpr float kCostTop = 48.0f;        // log2 cost mapped to bucket 0
+constexpr int kBucketsPerOctave = 16;
+constexpr int kCostBuckets = 48 * 16;
 
 }  // namespace cfamd

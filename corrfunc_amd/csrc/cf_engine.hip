@@ -15,11 +15,14 @@
 #include <string.h>
 #include <sys/time.h>
 
+#include <algorithm>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "cf_kernels.hip"
+#include "cf_sched.hip"
+#include "cf_model.hip"
 #include "../../include/corrfunc_theory.h"
 #include "../../include/corrfunc_amd.h"
 
@@ -62,15 +65,32 @@ struct Arena {
     }
 };
 
+// The work-list cost model: network parameters on the device, and the store
+// of recorded (features, log2 ticks) training samples.
+struct CostModel {
+    bool trained = false;
+    float* d_params = nullptr;  // kNumParams
+    float* d_norm = nullptr;    // kNumNorm
+    bool recording = false;
+    int cap = 1 << 21;          // samples kept in the store
+    float* d_sx = nullptr;      // [cap][kNumFeatures]
+    float* d_sy = nullptr;      // [cap]
+    int* d_count = nullptr;     // samples appended so far (may exceed cap)
+};
+
 struct Engine {
     bool init = false;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_list = nullptr;
     Arena arena;
     int engine_kind = CF_AMD_ENGINE_TUNED;
+    int sched_kind = CF_AMD_SCHED_LEARNED;
     cf_amd_stats last;
-    int waves_per_cu = 8;
+    int ncu = 256;
+    double ns_per_tick = 10.0;  // device wall clock period
+    CostModel model;
+    DevLattice hlats[2];
 };
 
 static Engine& eng()
@@ -81,8 +101,19 @@ static Engine& eng()
         CF_CHECK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
         CF_CHECK(hipEventCreate(&E.ev0));
         CF_CHECK(hipEventCreate(&E.ev1));
+        CF_CHECK(hipEventCreate(&E.ev_list));
         const char* k = getenv("CORRFUNC_AMD_ENGINE");
         if (k && strcmp(k, "plain") == 0) E.engine_kind = CF_AMD_ENGINE_PLAIN;
+        if (const char* sk = getenv("CORRFUNC_AMD_SCHED")) {
+            if (strcmp(sk, "none") == 0) E.sched_kind = CF_AMD_SCHED_NONE;
+            else if (strcmp(sk, "heuristic") == 0) E.sched_kind = CF_AMD_SCHED_HEURISTIC;
+        }
+        hipDeviceProp_t prop;
+        CF_CHECK(hipGetDeviceProperties(&prop, E.device));
+        E.ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E.device) == hipSuccess && khz > 0)
+            E.ns_per_tick = 1e6 / khz;
         memset(&E.last, 0, sizeof E.last);
         E.init = true;
     }
@@ -366,9 +397,13 @@ struct CallSpec {
     int nmu_bins;
     double boxsize;     // xi / wp
     int numthreads;
+    int cell_timer = 0; // record per-cell-pair timings (wp's c_cell_timer)
 };
 
+static_assert(sizeof(TimingRow) == sizeof(struct api_cell_timings), "timing rows must match api_cell_timings");
+
 struct CallOut {
+    std::vector<TimingRow> timings;  // per cell pair, when the cell timer is on
     std::vector<double> rupp;
     int nbin = 0, n2 = 0;
     std::vector<uint64_t> np;
@@ -550,6 +585,168 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
 
 }
 
+// Device copy of the (primary, secondary) lattice pair of a single call.
+static const DevLattice* lats_for(const DevLattice& L1, const DevLattice& L2)
+{
+    Engine& E = eng();
+    E.hlats[0] = L1;  // the engine keeps the host copy alive until the copy has run
+    E.hlats[1] = L2;
+    DevLattice* lats = (DevLattice*)E.arena.get("lats", sizeof(E.hlats));
+    CF_CHECK(hipMemcpyAsync(lats, E.hlats, sizeof(E.hlats), hipMemcpyHostToDevice, E.stream));
+    return lats;
+}
+
+// ---------------------------------------------------------- tuned launch
+constexpr int kSplitPerWave = 4;  // a work item is split when it exceeds 1/(4 * resident waves) of the total cost
+
+struct TunedRun {
+    int nitems = 0;
+    unsigned long long* ticks = nullptr;  // per cell pair wall-clock ticks (device), when timed
+    int* cu = nullptr;                    // compute unit of the last item of each cell pair (device)
+};
+
+// Samples store of the cost model, allocated on first use.
+static void ensure_sample_store()
+{
+    Engine& E = eng();
+    CostModel& M = E.model;
+    if (M.d_sx) return;
+    CF_CHECK(hipMalloc(&M.d_sx, sizeof(float) * (size_t)M.cap * kNumFeatures));
+    CF_CHECK(hipMalloc(&M.d_sy, sizeof(float) * (size_t)M.cap));
+    CF_CHECK(hipMalloc(&M.d_count, sizeof(int)));
+    k_fill_i32<<<1, 64, 0, E.stream>>>(1, M.d_count, 0);
+    CF_CHECK(hipGetLastError());
+}
+
+// Builds the work list of `ncp` cell pairs (ordered and split by the selected
+// scheduler) and launches the tuned pair kernel on it. `batch`: cell pair k
+// belongs to catalog cps[k].pad, whose lattice is lats[pad] and whose histogram
+// is g_*[pad * nslots]. Records E.ev_list before the work list, E.ev0 / E.ev1
+// around the pair kernel. With `timed`, per-cell-pair ticks are recorded (and
+// appended to the cost model's samples when recording is on).
+static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats, int batch, PairParams& P, int nslots,
+                        int nbin, const double* d_edges, const unsigned short* d_lut, unsigned long long* g_np,
+                        double* g_avg, double* g_w, bool timed, TunedRun& T)
+{
+    Engine& E = eng();
+    const int threads = 256, nwaves = threads / 64;
+    const int regions = batch ? nwaves : 1;
+    // histogram replication: as many copies (power of 2, <= 32) as fit a 40 KiB budget
+    const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
+    int R = 1;
+    while (R < 32 && (size_t)regions * nslots * per_slot * (2 * R) <= 40 * 1024) R *= 2;
+    if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
+        const int v = atoi(rc);
+        if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
+    }
+    P.hist_copies = R;
+    const size_t lds = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
+                       (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueue * 5;
+    if (lds > 64 * 1024) {
+        fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
+        return EXIT_FAILURE;
+    }
+    int per_cu = (int)((160 * 1024) / (lds + 1024));
+    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    const long max_blocks = (long)E.ncu * per_cu;
+
+    // ---- cost prediction and order
+    CF_CHECK(hipEventRecord(E.ev_list, E.stream));
+    const int nb = (ncp + 255) / 256;
+    const int sched = E.sched_kind;
+    const bool learned = sched == CF_AMD_SCHED_LEARNED && E.model.trained;
+    const bool record = timed && E.model.recording;
+    float* feats = nullptr;
+    if (learned || record) {
+        feats = (float*)E.arena.get("s_feats", sizeof(float) * kNumFeatures * (size_t)ncp);
+        k_cp_features<<<nb, 256, 0, E.stream>>>(ncp, cps, lats, batch, P, feats);
+        CF_CHECK(hipGetLastError());
+    }
+    float* cost = nullptr;
+    int *order = nullptr, *nparts = nullptr;
+    double* total = nullptr;
+    if (sched != CF_AMD_SCHED_NONE) {
+        cost = (float*)E.arena.get("s_cost", sizeof(float) * ncp);
+        if (learned)
+            k_mlp_predict<<<nb, 256, 0, E.stream>>>(ncp, feats, E.model.d_norm, E.model.d_params, cost);
+        else
+            k_cost_heuristic<<<nb, 256, 0, E.stream>>>(ncp, cps, lats, batch, cost);
+        CF_CHECK(hipGetLastError());
+        int* bcount = (int*)E.arena.get("s_bcount", sizeof(int) * (kCostBuckets + 1));
+        int* boff = (int*)E.arena.get("s_boff", sizeof(int) * (kCostBuckets + 1));
+        int* cursor = (int*)E.arena.get("s_cursor", sizeof(int) * kCostBuckets);
+        total = (double*)E.arena.get("s_total", sizeof(double));
+        k_fill_i32<<<4, 256, 0, E.stream>>>(kCostBuckets, bcount, 0);
+        k_fill_i32<<<4, 256, 0, E.stream>>>(kCostBuckets, cursor, 0);
+        k_fill_u64<<<1, 64, 0, E.stream>>>(1, (unsigned long long*)total, 0ull);  // +0.0
+        k_cost_bucket<<<nb, 256, 0, E.stream>>>(ncp, cost, bcount, total);
+        CF_CHECK(hipGetLastError());
+        device_scan(bcount, boff, kCostBuckets, "s_scan");
+        order = (int*)E.arena.get("s_order", sizeof(int) * ncp);
+        k_cost_scatter<<<nb, 256, 0, E.stream>>>(ncp, cost, boff, cursor, order);
+        CF_CHECK(hipGetLastError());
+        nparts = (int*)E.arena.get("s_nparts", sizeof(int) * ncp);
+    }
+
+    // ---- work items
+    int* nch = (int*)E.arena.get("wi_n", sizeof(int) * (ncp + 1));
+    int* off = (int*)E.arena.get("wi_off", sizeof(int) * (ncp + 1));
+    const double cap_div = (double)max_blocks * nwaves * kSplitPerWave;
+    k_count_chunks<<<nb, 256, 0, E.stream>>>(ncp, order, cps, lats, batch, cost, total, cap_div, nparts, nch);
+    CF_CHECK(hipGetLastError());
+    device_scan(nch, off, ncp, "wi_scan");
+    int nitems = 0;
+    CF_CHECK(hipMemcpyAsync(&nitems, off + ncp, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * ((size_t)nitems + 1));
+    k_emit_chunks<<<nb, 256, 0, E.stream>>>(ncp, order, cps, lats, batch, P, off, nparts, items);
+    CF_CHECK(hipGetLastError());
+    int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
+    k_fill_i32<<<1, 64, 0, E.stream>>>(1, counter, 0);
+    const int* d_nitems = off + ncp;
+    long blocks = max_blocks;
+    const long needed = ((long)nitems + nwaves - 1) / nwaves;
+    if (blocks > needed) blocks = needed > 0 ? needed : 1;
+
+    unsigned long long* ticks = nullptr;
+    int* cu = nullptr;
+    if (timed) {
+        ticks = (unsigned long long*)E.arena.get("t_ticks", sizeof(unsigned long long) * ncp);
+        cu = (int*)E.arena.get("t_cu", sizeof(int) * ncp);
+        k_fill_u64<<<nb, 256, 0, E.stream>>>(ncp, ticks, 0ull);
+        k_fill_i32<<<nb, 256, 0, E.stream>>>(ncp, cu, -1);
+    }
+
+    const void* kfn = nullptr;
+#define CF_K(M, F, B) if (P.mode == M && P.r2form == F && batch == B) kfn = (const void*)&k_pairs_tuned<M, F, B>;
+    CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
+    CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
+    CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
+    CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
+    CF_K(MODE_DD, R2_SUM_XY_Z, 1) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 1)
+#undef CF_K
+    if (!kfn) {
+        fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d%s\n", P.mode, P.r2form,
+                batch ? " (batched)" : "");
+        return EXIT_FAILURE;
+    }
+    void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w, &ticks, &cu};
+    CF_CHECK(hipEventRecord(E.ev0, E.stream));
+    CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
+    CF_CHECK(hipGetLastError());
+    CF_CHECK(hipEventRecord(E.ev1, E.stream));
+    if (record) {
+        ensure_sample_store();
+        k_append_samples<<<nb, 256, 0, E.stream>>>(ncp, feats, ticks, E.model.cap, E.model.d_count, E.model.d_sx,
+                                                   E.model.d_sy);
+        CF_CHECK(hipGetLastError());
+    }
+    T.nitems = nitems;
+    T.ticks = ticks;
+    T.cu = cu;
+    return EXIT_SUCCESS;
+}
+
 static int run_core(const CallSpec& c, struct config_options* o, struct extra_options* extra, CallOut& R)
 {
     Engine& E = eng();
@@ -721,69 +918,22 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     }
 
     int nitems = 0;
-    CF_CHECK(hipEventRecord(E.ev0, E.stream));
-    if (ncp > 0) {
-        if (E.engine_kind == CF_AMD_ENGINE_PLAIN) {
+    TunedRun T;
+    const bool tuned = ncp > 0 && E.engine_kind != CF_AMD_ENGINE_PLAIN;
+    if (tuned) {
+        if (launch_tuned(ncp, cps, lats_for(L1, L2), 0, P, (int)nslots, nbin, d_edges, d_lut, g_np, g_avg, g_w,
+                         c.cell_timer || E.model.recording, T) != EXIT_SUCCESS)
+            return EXIT_FAILURE;
+        nitems = T.nitems;
+    } else {
+        CF_CHECK(hipEventRecord(E.ev_list, E.stream));
+        CF_CHECK(hipEventRecord(E.ev0, E.stream));
+        if (ncp > 0) {
             k_pairs_plain<<<ncp, 128, lds_hist, E.stream>>>(P, L1, L2, cps, d_edges, g_np, g_avg, g_w);
             CF_CHECK(hipGetLastError());
-        } else {
-            DevLattice hl[2] = {L1, L2};
-            DevLattice* lats = (DevLattice*)E.arena.get("lats", sizeof(hl));
-            CF_CHECK(hipMemcpyAsync(lats, hl, sizeof(hl), hipMemcpyHostToDevice, E.stream));
-            int* nch = (int*)E.arena.get("wi_n", sizeof(int) * (ncp + 1));
-            int* off = (int*)E.arena.get("wi_off", sizeof(int) * (ncp + 1));
-            k_count_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, lats, 0, nch);
-            CF_CHECK(hipGetLastError());
-            device_scan(nch, off, ncp, "wi_scan");
-            CF_CHECK(hipMemcpyAsync(&nitems, off + ncp, sizeof(int), hipMemcpyDeviceToHost, E.stream));
-            CF_CHECK(hipStreamSynchronize(E.stream));
-            WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * (nitems + 1));
-            k_emit_chunks<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, lats, 0, P, off, items);
-            CF_CHECK(hipGetLastError());
-            int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
-            CF_CHECK(hipMemsetAsync(counter, 0, sizeof(int), E.stream));
-            const int* d_nitems = off + ncp;
-            const int threads = 256;
-            // histogram replication: as many copies (power of 2, <= 32) as fit a 40 KiB budget
-            const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
-            int R = 1;
-            while (R < 32 && (size_t)nslots * per_slot * (2 * R) <= 40 * 1024) R *= 2;
-            if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
-                const int v = atoi(rc);
-                if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
-            }
-            P.hist_copies = R;
-            const size_t lds = (size_t)nslots * per_slot * R + (size_t)nbin * 8 +
-                               (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)(threads / 64) * kQueue * 5;
-            if (lds > 64 * 1024) {
-                fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
-                return EXIT_FAILURE;
-            }
-            int ncu = 256;
-            hipDeviceProp_t prop;
-            if (hipGetDeviceProperties(&prop, E.device) == hipSuccess) ncu = prop.multiProcessorCount;
-            int per_cu = (int)((160 * 1024) / (lds + 1024));
-            per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
-            long blocks = (long)ncu * per_cu;
-            const long needed = (nitems + 3) / 4;
-            if (blocks > needed) blocks = needed > 0 ? needed : 1;
-            const void* kfn = nullptr;
-#define CF_K(M, F) if (P.mode == M && P.r2form == F) kfn = (const void*)&k_pairs_tuned<M, F, false>;
-            CF_K(MODE_DD, R2_SUM_XY_Z) CF_K(MODE_DD, R2_FMA_DZ_DY_DX)
-            CF_K(MODE_XI, R2_SUM_XY_Z) CF_K(MODE_XI, R2_X_SUM_YZ) CF_K(MODE_XI, R2_FMA_DZ_DY_DX)
-            CF_K(MODE_RPPI, RP_SUM) CF_K(MODE_RPPI, RP_FMA) CF_K(MODE_WP, RP_SUM) CF_K(MODE_WP, RP_FMA)
-            CF_K(MODE_SMU, R2_SUM_XY_Z) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ)
-#undef CF_K
-            if (!kfn) {
-                fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d\n", P.mode, P.r2form);
-                return EXIT_FAILURE;
-            }
-            void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w};
-            CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
-            CF_CHECK(hipGetLastError());
         }
+        CF_CHECK(hipEventRecord(E.ev1, E.stream));
     }
-    CF_CHECK(hipEventRecord(E.ev1, E.stream));
     R.np.assign(nslots, 0);
     R.avg.assign(nslots, 0.0);
     R.wavg.assign(nslots, 0.0);
@@ -801,10 +951,24 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         R.wsum = h[0];
         R.wsq = h[1];
     }
+    if (c.cell_timer && tuned) {
+        // per-cell-pair timings in the order of the cell-pair list (wp_impl:433-463)
+        TimingRow* rows = (TimingRow*)E.arena.get("t_rows", sizeof(TimingRow) * ncp);
+        k_timing_rows<<<(ncp + 255) / 256, 256, 0, E.stream>>>(ncp, cps, lats_for(L1, L2), 0, T.ticks, T.cu,
+                                                               E.ns_per_tick, rows);
+        CF_CHECK(hipGetLastError());
+        R.timings.resize(ncp);
+        CF_CHECK(hipMemcpyAsync(R.timings.data(), rows, sizeof(TimingRow) * ncp, hipMemcpyDeviceToHost, E.stream));
+    }
     CF_CHECK(hipStreamSynchronize(E.stream));
-    float kms = 0;
+    float kms = 0, lms = 0;
     CF_CHECK(hipEventElapsedTime(&kms, E.ev0, E.ev1));
+    CF_CHECK(hipEventElapsedTime(&lms, E.ev_list, E.ev0));
 
+    st.list_ms = lms;
+    st.sched = tuned ? (E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC
+                                                                                  : E.sched_kind)
+                     : CF_AMD_SCHED_NONE;
     st.nmesh[0] = g.nx; st.nmesh[1] = g.ny; st.nmesh[2] = g.nz;
     st.refine[0] = refine[0]; st.refine[1] = refine[1]; st.refine[2] = refine[2];
     st.num_cell_pairs = ncp;
@@ -941,46 +1105,17 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     CF_CHECK(hipMemcpyAsync(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice, E.stream));
 
     int nitems = 0;
-    CF_CHECK(hipEventRecord(E.ev0, E.stream));
+    TunedRun T;
     if (ncp_all > 0) {
-        int* nch = (int*)E.arena.get("wi_n", sizeof(int) * (ncp_all + 1));
-        int* off = (int*)E.arena.get("wi_off", sizeof(int) * (ncp_all + 1));
-        k_count_chunks<<<(ncp_all + 255) / 256, 256, 0, E.stream>>>(ncp_all, cps, lats, 1, nch);
-        CF_CHECK(hipGetLastError());
-        device_scan(nch, off, ncp_all, "wi_scan");
-        CF_CHECK(hipMemcpyAsync(&nitems, off + ncp_all, sizeof(int), hipMemcpyDeviceToHost, E.stream));
-        CF_CHECK(hipStreamSynchronize(E.stream));
-        WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * (nitems + 1));
-        k_emit_chunks<<<(ncp_all + 255) / 256, 256, 0, E.stream>>>(ncp_all, cps, lats, 1, P, off, items);
-        CF_CHECK(hipGetLastError());
-        int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
-        CF_CHECK(hipMemsetAsync(counter, 0, sizeof(int), E.stream));
-        const int* d_nitems = off + ncp_all;
-        const int threads = 256, nwaves = threads / 64;
-        const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
-        int R = 1;
-        while (R < 32 && (size_t)nwaves * nslots * per_slot * (2 * R) <= 40 * 1024) R *= 2;
-        P.hist_copies = R;
-        const size_t lds = (size_t)nwaves * nslots * per_slot * R + (size_t)nbin * 8 +
-                           (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueue * 5;
-        if (lds > 64 * 1024) {
-            fprintf(stderr, "Error: LDS requirement %zu B of the batched pair kernel exceeds 64 KiB\n", lds);
+        if (launch_tuned(ncp_all, cps, lats, 1, P, nslots, nbin, d_edges, d_lut, g_np, g_avg, g_w,
+                         E.model.recording, T) != EXIT_SUCCESS)
             return EXIT_FAILURE;
-        }
-        int ncu = 256;
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, E.device) == hipSuccess) ncu = prop.multiProcessorCount;
-        int per_cu = (int)((160 * 1024) / (lds + 1024));
-        per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
-        long blocks = (long)ncu * per_cu;
-        const long needed = (nitems + 3) / 4;
-        if (blocks > needed) blocks = needed > 0 ? needed : 1;
-        const void* kfn = P.r2form == R2_FMA_DZ_DY_DX ? (const void*)&k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true>
-                                                     : (const void*)&k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true>;
-        void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w};
-        CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
+        nitems = T.nitems;
+    } else {
+        CF_CHECK(hipEventRecord(E.ev_list, E.stream));
+        CF_CHECK(hipEventRecord(E.ev0, E.stream));
+        CF_CHECK(hipEventRecord(E.ev1, E.stream));
     }
-    CF_CHECK(hipEventRecord(E.ev1, E.stream));
     std::vector<uint64_t> np(tot);
     std::vector<double> avg(tot), wv(tot);
     CF_CHECK(hipMemcpyAsync(np.data(), g_np, sizeof(uint64_t) * tot, hipMemcpyDeviceToHost, E.stream));
@@ -996,8 +1131,9 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         CF_CHECK(hipMemcpyAsync(wsums.data(), ws, sizeof(double) * 2 * ncat, hipMemcpyDeviceToHost, E.stream));
     }
     CF_CHECK(hipStreamSynchronize(E.stream));
-    float kms = 0;
+    float kms = 0, lms = 0;
     CF_CHECK(hipEventElapsedTime(&kms, E.ev0, E.ev1));
+    CF_CHECK(hipEventElapsedTime(&lms, E.ev_list, E.ev0));
     outs.assign(ncat, CallOut());
     for (int k = 0; k < ncat; k++) {
         CallOut& R = outs[k];
@@ -1019,7 +1155,143 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     st.grid_ms = (t_grid - t_start) * 1e3;
     st.total_ms = (now_s() - t_start) * 1e3;
     st.engine = CF_AMD_ENGINE_TUNED;
+    st.list_ms = lms;
+    st.sched = E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC : E.sched_kind;
     E.last = st;
+    return EXIT_SUCCESS;
+}
+
+// ------------------------------------------------------------ cost model
+static_assert(CF_AMD_MODEL_FEATURES == kNumFeatures && CF_AMD_MODEL_PARAMS == kNumParams &&
+                  CF_AMD_MODEL_NORM == kNumNorm, "cost model sizes of corrfunc_amd.h");
+
+static void model_alloc()
+{
+    CostModel& M = eng().model;
+    if (M.d_params) return;
+    CF_CHECK(hipMalloc(&M.d_params, sizeof(float) * kNumParams));
+    CF_CHECK(hipMalloc(&M.d_norm, sizeof(float) * kNumNorm));
+}
+
+// splitmix64 (Steele, Lea & Flood 2014) and Box-Muller normals, for seeded
+// weight initialisation and minibatch permutations.
+struct Rng {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() { return sqrt(-2.0 * log(uniform())) * cos(6.283185307179586 * uniform()); }
+};
+
+// Standardisation of features and targets, from the samples (FP64 on the host).
+static std::vector<float> model_norm(const std::vector<float>& hx, const std::vector<float>& hy, int64_t n)
+{
+    std::vector<float> norm(kNumNorm);
+    for (int f = 0; f < kNumFeatures; f++) {
+        double m = 0, v = 0;
+        for (int64_t k = 0; k < n; k++) m += hx[(size_t)k * kNumFeatures + f];
+        m /= (double)n;
+        for (int64_t k = 0; k < n; k++) {
+            const double d = hx[(size_t)k * kNumFeatures + f] - m;
+            v += d * d;
+        }
+        const double sd = sqrt(v / (double)n);
+        norm[f] = (float)m;
+        norm[kNumFeatures + f] = sd > 1e-6 ? (float)(1.0 / sd) : 0.f;
+    }
+    double m = 0, v = 0;
+    for (int64_t k = 0; k < n; k++) m += hy[k];
+    m /= (double)n;
+    for (int64_t k = 0; k < n; k++) v += (hy[k] - m) * (hy[k] - m);
+    const double sd = sqrt(v / (double)n);
+    norm[2 * kNumFeatures] = (float)m;
+    norm[2 * kNumFeatures + 1] = sd > 1e-6 ? (float)sd : 1.f;
+    return norm;
+}
+
+// Loss and gradient of `nb` samples (rows idx[0..nb), or 0..nb) into d_grad /
+// d_loss, with the loss scaled by 1/inv_n.
+static void model_grad_dev(int nb, const int* idx, const float* dX, const float* dY, const float* d_norm,
+                           const float* d_params, float inv_n, float* d_grad, float* d_loss)
+{
+    Engine& E = eng();
+    const int nblk = (nb + kGradBlock - 1) / kGradBlock;
+    float* partial = (float*)E.arena.get("m_partial", sizeof(float) * (size_t)nblk * kNumParams);
+    float* ploss = (float*)E.arena.get("m_ploss", sizeof(float) * (size_t)nblk);
+    k_mlp_grad<<<nblk, kGradBlock, 0, E.stream>>>(nb, idx, dX, dY, d_norm, d_params, inv_n, partial, ploss);
+    CF_CHECK(hipGetLastError());
+    k_mlp_reduce<<<(kNumParams + 1 + 255) / 256, 256, 0, E.stream>>>(nblk, partial, ploss, d_grad, d_loss);
+    CF_CHECK(hipGetLastError());
+}
+
+static int model_train_dev(const float* dX, const float* dY, int64_t n, int epochs, int batch, float lr,
+                           unsigned seed, float* loss_out)
+{
+    Engine& E = eng();
+    CostModel& M = E.model;
+    if (n < 2 || epochs < 1 || batch < 1 || !(lr > 0)) {
+        fprintf(stderr, "Error: cost model training needs n >= 2 samples, epochs >= 1, batch >= 1, lr > 0\n");
+        return EXIT_FAILURE;
+    }
+    if (n > (1 << 30)) {
+        fprintf(stderr, "Error: too many training samples (%lld)\n", (long long)n);
+        return EXIT_FAILURE;
+    }
+    model_alloc();
+    std::vector<float> hx((size_t)n * kNumFeatures), hy((size_t)n);
+    CF_CHECK(hipMemcpyAsync(hx.data(), dX, sizeof(float) * hx.size(), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(hy.data(), dY, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    const std::vector<float> norm = model_norm(hx, hy, n);
+    // He-normal initialisation of the ReLU layers, zero biases
+    Rng rng{0x5eedull ^ ((uint64_t)seed << 17)};
+    std::vector<float> w(kNumParams, 0.f);
+    for (int k = 0; k < kHidden * kNumFeatures; k++) w[kOffW1 + k] = (float)(rng.normal() * sqrt(2.0 / kNumFeatures));
+    for (int k = 0; k < kHidden * kHidden; k++) w[kOffW2 + k] = (float)(rng.normal() * sqrt(2.0 / kHidden));
+    for (int k = 0; k < kHidden; k++) w[kOffW3 + k] = (float)(rng.normal() * sqrt(1.0 / kHidden));
+    float* d_m = (float*)E.arena.get("m_adam_m", sizeof(float) * kNumParams);
+    float* d_v = (float*)E.arena.get("m_adam_v", sizeof(float) * kNumParams);
+    float* d_grad = (float*)E.arena.get("m_grad", sizeof(float) * kNumParams);
+    float* d_loss = (float*)E.arena.get("m_loss", sizeof(float));
+    int* d_idx = (int*)E.arena.get("m_idx", sizeof(int) * (size_t)n);
+    CF_CHECK(hipMemcpyAsync(M.d_params, w.data(), sizeof(float) * kNumParams, hipMemcpyHostToDevice, E.stream));
+    CF_CHECK(hipMemcpyAsync(M.d_norm, norm.data(), sizeof(float) * kNumNorm, hipMemcpyHostToDevice, E.stream));
+    k_fill_i32<<<8, 256, 0, E.stream>>>(kNumParams, (int*)d_m, 0);  // +0.0f
+    k_fill_i32<<<8, 256, 0, E.stream>>>(kNumParams, (int*)d_v, 0);
+    std::vector<int> perm((size_t)n);
+    for (int64_t k = 0; k < n; k++) perm[k] = (int)k;
+    int step = 0;
+    for (int e = 0; e < epochs; e++) {
+        for (int64_t k = n - 1; k > 0; k--) std::swap(perm[k], perm[rng.next() % (uint64_t)(k + 1)]);
+        CF_CHECK(hipMemcpyAsync(d_idx, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, E.stream));
+        for (int64_t b0 = 0; b0 < n; b0 += batch) {
+            const int nb = (int)std::min<int64_t>(batch, n - b0);
+            model_grad_dev(nb, d_idx + b0, dX, dY, M.d_norm, M.d_params, 1.f / nb, d_grad, d_loss);
+            k_adam<<<(kNumParams + 255) / 256, 256, 0, E.stream>>>(M.d_params, d_grad, d_m, d_v, ++step, lr, 0.9f,
+                                                                  0.999f, 1e-8f);
+            CF_CHECK(hipGetLastError());
+        }
+        CF_CHECK(hipStreamSynchronize(E.stream));  // perm is rewritten next epoch
+    }
+    // loss over all samples, in chunks
+    double total = 0;
+    const int64_t chunk = 1 << 16;
+    for (int64_t b0 = 0; b0 < n; b0 += chunk) {
+        const int nb = (int)std::min<int64_t>(chunk, n - b0);
+        model_grad_dev(nb, nullptr, dX + b0 * kNumFeatures, dY + b0, M.d_norm, M.d_params, (float)(1.0 / n), d_grad,
+                       d_loss);
+        float l = 0;
+        CF_CHECK(hipMemcpyAsync(&l, d_loss, sizeof(float), hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        total += l;
+    }
+    if (loss_out) *loss_out = (float)total;
+    M.trained = true;
     return EXIT_SUCCESS;
 }
 
@@ -1296,8 +1568,25 @@ static int xi_wp_run(int mode, const int64_t ND, void* X, void* Y, void* Z, doub
     fix_maxcells(options);
     fix_refine(options);
     CallSpec c{mode, ND, 0, X, Y, Z, nullptr, nullptr, nullptr, 1, binfile, pimax, 0, 0, boxsize, numthreads};
+    c.cell_timer = mode == MODE_WP && options->c_cell_timer;
     if (run_core(c, options, extra, R) != EXIT_SUCCESS) return EXIT_FAILURE;
     finalize(R, 1, ND, options->need_avg_sep, 1);
+    if (c.cell_timer) {
+        // the reference's allocate_cell_timer (utils/defs.h:448-462): an existing
+        // array of at least num_cell_pairs rows is reused, else replaced
+        const int64_t ncp = (int64_t)R.timings.size();
+        if (options->totncells_timings < ncp) {
+            free_cell_timings(options);
+            options->cell_timings = (struct api_cell_timings*)calloc(ncp, sizeof(struct api_cell_timings));
+            if (options->cell_timings == NULL)
+                fprintf(stderr, "Warning: In %s> Could not allocate memory to store the API timings per cell. \n",
+                        "countpairs_wp");
+            else
+                options->totncells_timings = ncp;
+        }
+        if (options->cell_timings != NULL && options->totncells_timings >= ncp && ncp > 0)
+            memcpy(options->cell_timings, R.timings.data(), sizeof(struct api_cell_timings) * ncp);
+    }
     return EXIT_SUCCESS;
 }
 
@@ -1474,6 +1763,169 @@ int cf_amd_countpairs_batch(int ncat, const int64_t* ND, void** X, void** Y, voi
     });
 }
 
+int cf_amd_set_scheduler(int kind)
+{
+    return guarded([&]() -> int {
+        if (kind != CF_AMD_SCHED_NONE && kind != CF_AMD_SCHED_HEURISTIC && kind != CF_AMD_SCHED_LEARNED)
+            return EXIT_FAILURE;
+        eng().sched_kind = kind;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_get_scheduler(void) { return guarded([&]() -> int { return eng().sched_kind; }); }
+
+int cf_amd_record_costs(int on)
+{
+    return guarded([&]() -> int {
+        if (on) ensure_sample_store();
+        eng().model.recording = on != 0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int64_t cf_amd_cost_samples(float* X, float* y, int64_t cap)
+{
+    int64_t out = -1;
+    if (guarded([&]() -> int {
+            Engine& E = eng();
+            ensure_sample_store();
+            int count = 0;
+            CF_CHECK(hipMemcpyAsync(&count, E.model.d_count, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+            CF_CHECK(hipStreamSynchronize(E.stream));
+            const int64_t have = std::min<int64_t>(count, E.model.cap);
+            const int64_t k = std::min<int64_t>(have, cap > 0 ? cap : 0);
+            if (X && k > 0)
+                CF_CHECK(hipMemcpyAsync(X, E.model.d_sx, sizeof(float) * (size_t)k * kNumFeatures,
+                                        hipMemcpyDeviceToHost, E.stream));
+            if (y && k > 0)
+                CF_CHECK(hipMemcpyAsync(y, E.model.d_sy, sizeof(float) * (size_t)k, hipMemcpyDeviceToHost, E.stream));
+            CF_CHECK(hipStreamSynchronize(E.stream));
+            out = have;
+            return EXIT_SUCCESS;
+        }) != EXIT_SUCCESS)
+        return -1;
+    return out;
+}
+
+int cf_amd_clear_cost_samples(void)
+{
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        ensure_sample_store();
+        k_fill_i32<<<1, 64, 0, E.stream>>>(1, E.model.d_count, 0);
+        CF_CHECK(hipGetLastError());
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_model_train(const float* X, const float* y, int64_t n, int epochs, int batch, float lr, unsigned seed,
+                       float* loss)
+{
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        if (!X) {
+            ensure_sample_store();
+            int count = 0;
+            CF_CHECK(hipMemcpyAsync(&count, E.model.d_count, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+            CF_CHECK(hipStreamSynchronize(E.stream));
+            const int64_t have = std::min<int64_t>(count, E.model.cap);
+            return model_train_dev(E.model.d_sx, E.model.d_sy, have, epochs, batch, lr, seed, loss);
+        }
+        if (!y || n < 1) return EXIT_FAILURE;
+        float* dX = (float*)E.arena.get("m_tx", sizeof(float) * (size_t)n * kNumFeatures);
+        float* dY = (float*)E.arena.get("m_ty", sizeof(float) * (size_t)n);
+        CF_CHECK(hipMemcpyAsync(dX, X, sizeof(float) * (size_t)n * kNumFeatures, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipMemcpyAsync(dY, y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, E.stream));
+        return model_train_dev(dX, dY, n, epochs, batch, lr, seed, loss);
+    });
+}
+
+int cf_amd_model_get(float* params, float* norm)
+{
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        if (!E.model.trained) return EXIT_FAILURE;
+        if (params)
+            CF_CHECK(hipMemcpyAsync(params, E.model.d_params, sizeof(float) * kNumParams, hipMemcpyDeviceToHost,
+                                    E.stream));
+        if (norm)
+            CF_CHECK(hipMemcpyAsync(norm, E.model.d_norm, sizeof(float) * kNumNorm, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_model_set(const float* params, const float* norm)
+{
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        if (!params || !norm) return EXIT_FAILURE;
+        model_alloc();
+        CF_CHECK(hipMemcpyAsync(E.model.d_params, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice,
+                                E.stream));
+        CF_CHECK(hipMemcpyAsync(E.model.d_norm, norm, sizeof(float) * kNumNorm, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        E.model.trained = true;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_model_reset(void)
+{
+    return guarded([&]() -> int {
+        eng().model.trained = false;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_model_trained(void) { return guarded([&]() -> int { return eng().model.trained ? 1 : 0; }); }
+
+int cf_amd_model_predict(const float* X, int64_t n, float* out)
+{
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        if (!E.model.trained || !X || !out || n < 0 || n > (1 << 30)) return EXIT_FAILURE;
+        if (n == 0) return EXIT_SUCCESS;
+        float* dX = (float*)E.arena.get("m_px", sizeof(float) * (size_t)n * kNumFeatures);
+        float* dO = (float*)E.arena.get("m_po", sizeof(float) * (size_t)n);
+        CF_CHECK(hipMemcpyAsync(dX, X, sizeof(float) * (size_t)n * kNumFeatures, hipMemcpyHostToDevice, E.stream));
+        k_mlp_predict<<<(unsigned)((n + 255) / 256), 256, 0, E.stream>>>((int)n, dX, E.model.d_norm,
+                                                                         E.model.d_params, dO);
+        CF_CHECK(hipGetLastError());
+        CF_CHECK(hipMemcpyAsync(out, dO, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_model_grad(const float* X, const float* y, int64_t n, const float* params, const float* norm,
+                      float* grad, float* loss)
+{
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        if (!X || !y || !params || !norm || !grad || n < 1 || n > (1 << 20)) return EXIT_FAILURE;
+        float* dX = (float*)E.arena.get("m_gx", sizeof(float) * (size_t)n * kNumFeatures);
+        float* dY = (float*)E.arena.get("m_gy", sizeof(float) * (size_t)n);
+        float* dP = (float*)E.arena.get("m_gp", sizeof(float) * kNumParams);
+        float* dN = (float*)E.arena.get("m_gn", sizeof(float) * kNumNorm);
+        float* dG = (float*)E.arena.get("m_gg", sizeof(float) * kNumParams);
+        float* dL = (float*)E.arena.get("m_gl", sizeof(float));
+        CF_CHECK(hipMemcpyAsync(dX, X, sizeof(float) * (size_t)n * kNumFeatures, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipMemcpyAsync(dY, y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipMemcpyAsync(dP, params, sizeof(float) * kNumParams, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipMemcpyAsync(dN, norm, sizeof(float) * kNumNorm, hipMemcpyHostToDevice, E.stream));
+        model_grad_dev((int)n, nullptr, dX, dY, dN, dP, (float)(1.0 / n), dG, dL);
+        CF_CHECK(hipMemcpyAsync(grad, dG, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, E.stream));
+        float l = 0;
+        CF_CHECK(hipMemcpyAsync(&l, dL, sizeof(float), hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        if (loss) *loss = l;
+        return EXIT_SUCCESS;
+    });
+}
+
 const char* cf_amd_version(void) { return "corrfunc_amd 0.1 (reference API 2.5.3, gfx950)"; }
 
 void* cf_amd_device_alloc(size_t bytes)
@@ -1500,9 +1952,25 @@ int cf_amd_memcpy(void* dst, const void* src, size_t bytes, int kind)
 {
     return guarded([&]() -> int {
         Engine& E = eng();
-        const hipMemcpyKind k = kind == CF_AMD_H2D ? hipMemcpyHostToDevice
-                              : kind == CF_AMD_D2H ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-        CF_CHECK(hipMemcpyAsync(dst, src, bytes, k, E.stream));
+        if (kind != CF_AMD_H2D && kind != CF_AMD_D2H && kind != CF_AMD_D2D) {
+            fprintf(stderr, "cf_amd_memcpy: unknown copy kind %d\n", kind);
+            return EXIT_FAILURE;
+        }
+        if (kind == CF_AMD_D2D && ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0)) {
+            // device copies run through the engine's own kernel (16-byte words + tail)
+            const int64_t n16 = (int64_t)(bytes / 16);
+            const int tail = (int)(bytes % 16);
+            const int64_t want = (n16 + 255) / 256;
+            const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 32));
+            k_copy16<<<blocks, 256, 0, E.stream>>>(n16, (const uint4*)src, (uint4*)dst, tail,
+                                                   (const unsigned char*)src + n16 * 16,
+                                                   (unsigned char*)dst + n16 * 16);
+            CF_CHECK(hipGetLastError());
+        } else {
+            const hipMemcpyKind k = kind == CF_AMD_H2D ? hipMemcpyHostToDevice
+                                  : kind == CF_AMD_D2H ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+            CF_CHECK(hipMemcpyAsync(dst, src, bytes, k, E.stream));
+        }
         CF_CHECK(hipStreamSynchronize(E.stream));
         return EXIT_SUCCESS;
     });

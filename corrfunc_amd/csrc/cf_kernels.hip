@@ -227,6 +227,17 @@ __global__ void k_weight_sums(int64_t n, const double* __restrict__ w, double* _
     }
 }
 
+// Device-to-device copy: n16 16-byte words through the vector path (grid-stride),
+// then the byte tail. Reads and writes exactly the bytes asked for, which also
+// makes it the reference kernel for scaling the HBM byte counters.
+__global__ void k_copy16(int64_t n16, const uint4* __restrict__ src, uint4* __restrict__ dst, int tail,
+                         const unsigned char* __restrict__ src_tail, unsigned char* __restrict__ dst_tail)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    if (blockIdx.x == 0 && (int)threadIdx.x < tail) dst_tail[threadIdx.x] = src_tail[threadIdx.x];
+}
+
 // ------------------------------------------------------- cell pairs (a6)
 struct NgbArgs {
     int nx, ny, nz;
@@ -751,6 +762,16 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     if (cp.same && jlo < i0 + 1) jlo = i0 + 1;
     jlo = __builtin_amdgcn_readfirstlane(jlo);
     jhi = __builtin_amdgcn_readfirstlane(jhi);
+    {
+        const int nparts = __builtin_amdgcn_readfirstlane(it.nparts);
+        if (nparts > 1 && jhi > jlo) {
+            const int part = __builtin_amdgcn_readfirstlane(it.part);
+            const int64_t len = jhi - jlo;
+            const int a = jlo + (int)(len * part / nparts), b = jlo + (int)(len * (part + 1) / nparts);
+            jlo = a;
+            jhi = b;
+        }
+    }
 
     if (dense) {
         const int same = cp.same;
@@ -842,7 +863,9 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
                                                      const double* __restrict__ rupp_sqr,
                                                      const unsigned short* __restrict__ lut_g,
                                                      unsigned long long* __restrict__ g_np,
-                                                     double* __restrict__ g_avg, double* __restrict__ g_w)
+                                                     double* __restrict__ g_avg, double* __restrict__ g_w,
+                                                     unsigned long long* __restrict__ cp_ticks,
+                                                     int* __restrict__ cp_cu)
 {
     extern __shared__ unsigned long long smem_u64[];
     KernelLds S;
@@ -898,7 +921,18 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
                 L2 = L1;
             }
         }
-        process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
+        if (cp_ticks) {
+            // per-cell-pair cost record: wall-clock ticks of the wave on this item
+            const unsigned long long t0 = wall_clock64();
+            process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
+            const unsigned long long dt = wall_clock64() - t0;
+            if (lane == 0) {
+                atomicAdd(&cp_ticks[it.cp], dt);
+                cp_cu[it.cp] = (int)__smid();
+            }
+        } else {
+            process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
+        }
     }
     if (BATCH) {
         if (cur >= 0) {
@@ -913,94 +947,130 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
 
 template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_XI, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_XI, R2_X_SUM_YZ, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_XI, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_RPPI, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_RPPI, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_WP, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_WP, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_SMU, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_SMU, R2_FMA_DX_DY_DZ, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*);
+    double*, unsigned long long*, int*);
 
-// Work-item construction: chunks of 64 primary particles per cell pair.
-__global__ void k_count_chunks(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
-                               int batch, int* __restrict__ nchunks)
+// Largest squared separation between the bounding boxes of a cell pair
+// (primary box shifted by the wrap), per the geometry of the mode; the pair is
+// dense when every pair of its particles is inside the range.
+__device__ __forceinline__ int cp_dense(const PairParams& P, const CellPairRec& cp, const double* b1,
+                                        const double* b2, double* maxsep2)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncp) return;
-    const int* bs1 = lats[batch ? cps[t].pad : 0].bstart;
-    const int n = (int)cell_count(bs1, cps[t].c1);
-    nchunks[t] = (n + 63) / 64;
-}
-
-__global__ void k_emit_chunks(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
-                              int batch, PairParams P, const int* __restrict__ off, WorkItem* __restrict__ items)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncp) return;
-    const CellPairRec cp = cps[t];
-    const int* bs1 = lats[batch ? cp.pad : 0].bstart;
-    const double* bd1 = lats[batch ? cp.pad : 0].bounds;
-    const double* bd2 = lats[batch ? cp.pad : 1].bounds;
-    const int c1 = cp.c1;
-    // Largest separation between the two cells' bounding boxes (primary box
-    // shifted by the wrap): when it is inside the range, every pair is.
-    const double* b1 = bd1 + (int64_t)c1 * 6;
-    const double* b2 = bd2 + (int64_t)cp.c2 * 6;
     const double ox = cp.wx * P.xwrap, oy = cp.wy * P.ywrap, oz = cp.wz * P.zwrap;
     const double mx = fmax(fabs(b2[1] - (b1[0] + ox)), fabs((b1[1] + ox) - b2[0]));
     const double my = fmax(fabs(b2[3] - (b1[2] + oy)), fabs((b1[3] + oy) - b2[2]));
     const double mz = fmax(fabs(b2[5] - (b1[4] + oz)), fabs((b1[5] + oz) - b2[4]));
-    int dense;
-    if (P.mode == MODE_RPPI || P.mode == MODE_WP)
-        dense = mx * mx + my * my < 0.98 * P.sqr_rmax && mz < 0.98 * P.pimax;
-    else
-        dense = mx * mx + my * my + mz * mz < 0.98 * P.sqr_rmax;
-    const int s = bs1[(int64_t)c1 * kSlabs], e = bs1[(int64_t)(c1 + 1) * kSlabs];
+    if (P.mode == MODE_RPPI || P.mode == MODE_WP) {
+        *maxsep2 = mx * mx + my * my;
+        return mx * mx + my * my < 0.98 * P.sqr_rmax && mz < 0.98 * P.pimax;
+    }
+    *maxsep2 = mx * mx + my * my + mz * mz;
+    return mx * mx + my * my + mz * mz < 0.98 * P.sqr_rmax;
+}
+
+// A predicted log2 cost clamped to [0, kCostTop] (NaN -> 0), so that any model
+// output gives a valid bucket and a finite total.
+__device__ __forceinline__ float sane_cost(float c)
+{
+    return c >= 0.f ? (c <= kCostTop ? c : kCostTop) : 0.f;
+}
+
+// Work-item construction: chunks of 64 primary particles per cell pair, each
+// split into `parts` slices of its secondary window. Position t of the list
+// holds cell pair order[t] (order == nullptr: t). With a cost array (log2
+// units), a chunk whose predicted cost exceeds total / cap_div is split into
+// ceil(cost / cap) parts (at most kMaxParts).
+__global__ void k_count_chunks(int ncp, const int* __restrict__ order, const CellPairRec* __restrict__ cps,
+                               const DevLattice* __restrict__ lats, int batch, const float* __restrict__ cost,
+                               const double* __restrict__ total, double cap_div, int* __restrict__ nparts,
+                               int* __restrict__ nitems)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const int c = order ? order[t] : t;
+    const int* bs1 = lats[batch ? cps[c].pad : 0].bstart;
+    const int n = (int)cell_count(bs1, cps[c].c1);
+    const int nch = (n + 63) / 64;
+    int parts = 1;
+    if (cost && nch > 0) {
+        const double cap = *total / cap_div;
+        const double per_chunk = exp2((double)sane_cost(cost[c])) / nch;
+        if (cap > 0 && per_chunk > cap) parts = (int)fmin((double)kMaxParts, ceil(per_chunk / cap));
+    }
+    if (nparts) nparts[t] = parts;
+    nitems[t] = nch * parts;
+}
+
+__global__ void k_emit_chunks(int ncp, const int* __restrict__ order, const CellPairRec* __restrict__ cps,
+                              const DevLattice* __restrict__ lats, int batch, PairParams P,
+                              const int* __restrict__ off, const int* __restrict__ nparts,
+                              WorkItem* __restrict__ items)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const int c = order ? order[t] : t;
+    const CellPairRec cp = cps[c];
+    const int* bs1 = lats[batch ? cp.pad : 0].bstart;
+    const double* bd1 = lats[batch ? cp.pad : 0].bounds;
+    const double* bd2 = lats[batch ? cp.pad : 1].bounds;
+    double maxsep2;
+    const int dense = cp_dense(P, cp, bd1 + (int64_t)cp.c1 * 6, bd2 + (int64_t)cp.c2 * 6, &maxsep2);
+    const int s = bs1[(int64_t)cp.c1 * kSlabs], e = bs1[(int64_t)(cp.c1 + 1) * kSlabs];
+    const int parts = nparts ? nparts[t] : 1;
     int o = off[t];
     const int nch = (e - s + 63) / 64;
-    for (int c = 0; c < nch; c++) {
+    for (int ch = 0; ch < nch; ch++) {
         WorkItem w;
-        w.cp = t;
+        w.cp = c;
         if (dense && P.dense_stride && !cp.same) {
-            w.i0 = s + c;
+            w.i0 = s + ch;
             w.i1 = e;
             w.flags = kItemDense | (nch << kStrideShift);
         } else {
-            w.i0 = s + 64 * c;
+            w.i0 = s + 64 * ch;
             w.i1 = w.i0 + 64 < e ? w.i0 + 64 : e;
             w.flags = dense ? kItemDense : 0;
         }
-        items[o++] = w;
+        w.nparts = parts;
+        for (int p = 0; p < parts; p++) {
+            w.part = p;
+            items[o++] = w;
+        }
     }
 }
 

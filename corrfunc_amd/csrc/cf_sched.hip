@@ -1,0 +1,153 @@
+// cf_sched.hip -- cost-ordered work lists of the tuned pair kernel (gfx950).
+//
+// The persistent pair kernel pulls work items from a queue in list order, so
+// the order and the size of the items set its load balance. The scheduler
+//   1. computes kNumFeatures features per cell pair              (k_cp_features)
+//   2. predicts each cell pair's cost, in log2 units, either with the plain
+//      N1*N2 heuristic (k_cost_heuristic) or with the cost model's network
+//      (k_mlp_predict, cf_model.hip)
+//   3. sorts the cell pairs by descending cost with a counting sort over
+//      kCostBuckets buckets of 1/16 octave        (k_cost_bucket, k_cost_scatter)
+//   4. splits heavy cell pairs into parts of their secondary window
+//      (k_count_chunks / k_emit_chunks with the cost array).
+// The counts do not depend on the order or the split: every (i, j) pair is
+// still tested exactly once.
+#include "cf_device.h"
+
+namespace cfamd {
+
+__global__ void k_cp_features(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
+                              int batch, PairParams P, float* __restrict__ feats)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const CellPairRec cp = cps[t];
+    const DevLattice& A = lats[batch ? cp.pad : 0];
+    const DevLattice& B = lats[batch ? cp.pad : 1];
+    const double n1 = (double)cell_count(A.bstart, cp.c1), n2 = (double)cell_count(B.bstart, cp.c2);
+    const double* b1 = A.bounds + (int64_t)cp.c1 * 6;
+    const double* b2 = B.bounds + (int64_t)cp.c2 * 6;
+    double maxsep2;
+    const int dense = cp_dense(P, cp, b1, b2, &maxsep2);
+    const int is2d = P.mode == MODE_RPPI || P.mode == MODE_WP;
+    const double minsep2 = cp.mdx * cp.mdx + cp.mdy * cp.mdy + (is2d ? 0.0 : cp.mdz * cp.mdz);
+    const double oz = cp.wz * P.zwrap;
+    const double zlo = b1[4] + oz - P.zreach, zhi = b1[5] + oz + P.zreach;
+    const double ext2 = b2[5] - b2[4];
+    double zfrac = 1.0;
+    if (ext2 > 0) {
+        const double ov = fmin(zhi, b2[5]) - fmax(zlo, b2[4]);
+        zfrac = fmin(1.0, fmax(0.0, ov / ext2));
+    }
+    const double inv = P.sqr_rmax > 0 ? 1.0 / P.sqr_rmax : 0.0;
+    float* f = feats + (int64_t)t * kNumFeatures;
+    f[0] = (float)log2(1.0 + n1);
+    f[1] = (float)log2(1.0 + n2);
+    f[2] = (float)log2(fmax(1.0, ceil(n1 / 64.0)));
+    f[3] = (float)cp.same;
+    f[4] = (float)dense;
+    f[5] = (float)fmin(8.0, minsep2 * inv);
+    f[6] = (float)fmin(8.0, maxsep2 * inv);
+    f[7] = (float)zfrac;
+    f[8] = (float)is2d;
+    f[9] = (float)log2(1.0 + n2 * zfrac);
+}
+
+// The plain heuristic: log2 of the number of pairs tested, N1*N2 (N1(N1-1)/2
+// for a cell with itself).
+__global__ void k_cost_heuristic(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
+                                 int batch, float* __restrict__ cost)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const CellPairRec cp = cps[t];
+    const double n1 = (double)cell_count(lats[batch ? cp.pad : 0].bstart, cp.c1);
+    const double n2 = (double)cell_count(lats[batch ? cp.pad : 1].bstart, cp.c2);
+    const double pairs = cp.same ? 0.5 * n1 * (n1 - 1.0) : n1 * n2;
+    cost[t] = (float)log2(1.0 + pairs);
+}
+
+__device__ __forceinline__ int cost_bucket(float c)
+{
+    c = sane_cost(c);
+    int b = (int)((kCostTop - c) * (float)kBucketsPerOctave);
+    return b < 0 ? 0 : (b >= kCostBuckets ? kCostBuckets - 1 : b);
+}
+
+// Bucket histogram of the costs and the total cost sum(2^cost) (for the split cap).
+__global__ void k_cost_bucket(int ncp, const float* __restrict__ cost, int* __restrict__ counts,
+                              double* __restrict__ total)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (t < ncp) {
+        v = exp2((double)sane_cost(cost[t]));
+        atomicAdd(&counts[cost_bucket(cost[t])], 1);
+    }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(total, v);
+}
+
+__global__ void k_cost_scatter(int ncp, const float* __restrict__ cost, const int* __restrict__ bucket_off,
+                               int* __restrict__ cursor, int* __restrict__ order)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const int b = cost_bucket(cost[t]);
+    order[bucket_off[b] + atomicAdd(&cursor[b], 1)] = t;
+}
+
+// Integer and FP64 fills (graph-capturable, unlike memset nodes).
+__global__ void k_fill_i32(int64_t n, int* __restrict__ p, int v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(int64_t n, unsigned long long* __restrict__ p, unsigned long long v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Training samples of the cost model: the features and log2 of the recorded
+// ticks of every cell pair that took any time, appended to a bounded store.
+__global__ void k_append_samples(int ncp, const float* __restrict__ feats,
+                                 const unsigned long long* __restrict__ ticks, int cap, int* __restrict__ count,
+                                 float* __restrict__ sx, float* __restrict__ sy)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp || ticks[t] == 0) return;
+    const int k = atomicAdd(count, 1);
+    if (k >= cap) return;
+    for (int f = 0; f < kNumFeatures; f++) sx[(int64_t)k * kNumFeatures + f] = feats[(int64_t)t * kNumFeatures + f];
+    sy[k] = (float)log2((double)ticks[t]);
+}
+
+// One row of the per-cell-pair timer, laid out as the reference's
+// struct api_cell_timings (utils/defs.h:37-45).
+struct TimingRow {
+    int64_t N1, N2, time_in_ns;
+    int first_cellindex, second_cellindex, tid;
+    int pad;
+};
+static_assert(sizeof(TimingRow) == 40, "TimingRow is 40 bytes");
+
+__global__ void k_timing_rows(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
+                              int batch, const unsigned long long* __restrict__ ticks, const int* __restrict__ cu,
+                              double ns_per_tick, TimingRow* __restrict__ rows)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncp) return;
+    const CellPairRec cp = cps[t];
+    TimingRow r;
+    r.N1 = cell_count(lats[batch ? cp.pad : 0].bstart, cp.c1);
+    r.N2 = cell_count(lats[batch ? cp.pad : 1].bstart, cp.c2);
+    r.time_in_ns = (int64_t)((double)ticks[t] * ns_per_tick);
+    r.first_cellindex = cp.c1;
+    r.second_cellindex = cp.c2;
+    r.tid = cu[t];
+    r.pad = 0;
+    rows[t] = r;
+}
+
+}  // namespace cfamd

@@ -35,7 +35,7 @@ def wp(boxsize, pimax, nthreads, binfile, X, Y, Z,
     if c_api_timer:
         ret += (out["api_time"],)
     if c_cell_timer:
-        ret += (np.zeros(0, dtype=[("N1", np.int64), ("N2", np.int64), ("time_in_ns", np.int64),
-                                   ("first_cellindex", np.int32), ("second_cellindex", np.int32),
-                                   ("tid", np.int32)]),)
+        # one row per cell pair: N1, N2, time_in_ns (device wall clock of the
+        # waves on the pair), cellidx1, cellidx2, tid (compute unit)
+        ret += (out["cell_timings"],)
     return ret[0] if len(ret) == 1 else ret

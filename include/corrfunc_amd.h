@@ -39,8 +39,56 @@ typedef struct {
     double grid_ms;          /* host wall time from entry to the end of gridding */
     double total_ms;         /* host wall time of the engine part of the call */
     int engine;              /* CF_AMD_ENGINE_* used */
-    int pad;
+    int sched;               /* CF_AMD_SCHED_* that ordered the work list */
+    double list_ms;          /* cost prediction, ordering and work-list construction (device events) */
 } cf_amd_stats;
+
+/* Work-list schedulers of the TUNED kernel. Its persistent waves pull work items
+ * (<= 64 primaries of a cell pair, or a slice of their secondaries) in list order.
+ *   NONE       cell-pair list order, one item per 64 primaries;
+ *   HEURISTIC  cell pairs sorted by descending N1*N2, items heavier than
+ *              1/(4 * resident waves) of the total split into slices;
+ *   LEARNED    the same with the cost predicted by the cost model's network
+ *              (falls back to HEURISTIC while no model is trained or loaded).
+ * Counts are identical under every scheduler. Default LEARNED; the environment
+ * variable CORRFUNC_AMD_SCHED=none|heuristic selects another at load time. */
+#define CF_AMD_SCHED_NONE 0
+#define CF_AMD_SCHED_HEURISTIC 1
+#define CF_AMD_SCHED_LEARNED 2
+int cf_amd_set_scheduler(int kind);
+int cf_amd_get_scheduler(void);
+
+/* The cost model: an MLP (CF_AMD_MODEL_FEATURES inputs, two hidden layers of 32
+ * ReLU units, one output) that predicts log2 of the device time of a cell pair
+ * from features the engine computes per cell pair. Training samples are recorded
+ * by the engine itself: while recording is on, every TUNED call times each cell
+ * pair (wall clock of the waves on its items) and appends (features, log2 ticks)
+ * to a device store of CF_AMD_MODEL_STORE samples. Training (Adam on the MSE of
+ * standardised targets) and inference run as HIP kernels on the GPU.
+ * Parameter layout: W1[32][F] | b1[32] | W2[32][32] | b2[32] | W3[32] | b3;
+ * normalisation: mean[F] | inv_sd[F] | y_mean | y_sd. */
+#define CF_AMD_MODEL_FEATURES 10
+#define CF_AMD_MODEL_PARAMS 1441
+#define CF_AMD_MODEL_NORM 22
+#define CF_AMD_MODEL_STORE (1 << 21)
+int cf_amd_record_costs(int on);
+/* Copies up to `cap` recorded samples (X: [cap][F] row-major, y: log2 ticks);
+ * returns the number stored (X, y may be NULL to query it), or -1 on error. */
+int64_t cf_amd_cost_samples(float *X, float *y, int64_t cap);
+int cf_amd_clear_cost_samples(void);
+/* Trains from fresh weights (seeded) on n samples (X == NULL: the recorded
+ * ones); writes the final loss over all samples to *loss when non-NULL. */
+int cf_amd_model_train(const float *X, const float *y, int64_t n, int epochs, int batch, float lr, unsigned seed,
+                       float *loss);
+int cf_amd_model_get(float *params, float *norm);
+int cf_amd_model_set(const float *params, const float *norm);  /* installs a trained model */
+int cf_amd_model_reset(void);                                 /* forgets it */
+int cf_amd_model_trained(void);
+int cf_amd_model_predict(const float *X, int64_t n, float *out);  /* log2 cost of n feature rows */
+/* Loss L = 1/(2n) sum (net(x) - (y - y_mean)/y_sd)^2 of n <= 2^20 samples and its
+ * gradient with respect to `params` (the GPU's forward and backward kernels). */
+int cf_amd_model_grad(const float *X, const float *y, int64_t n, const float *params, const float *norm,
+                      float *grad, float *loss);
 
 /* Selects the pair kernel for subsequent calls; returns EXIT_SUCCESS/EXIT_FAILURE. */
 int cf_amd_set_engine(int kind);

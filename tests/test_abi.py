@@ -18,7 +18,7 @@ def declared_functions():
     for h in ("corrfunc_theory.h", "corrfunc_amd.h"):
         text = open(os.path.join(INC, h)).read()
         text = re.sub(r"static inline[^{]*\{(?:[^{}]|\{[^{}]*\})*\}", "", text)
-        names += re.findall(r"^\s*(?:int|void|const char\s*\*)\s+\**\s*(\w+)\s*\(", text, re.M)
+        names += re.findall(r"^\s*(?:int64_t|int|void|const char\s*\*)\s+\**\s*(\w+)\s*\(", text, re.M)
     return sorted(set(names))
 
 

@@ -122,3 +122,15 @@ def test_batch_matches_single_calls_and_oracle(engine, oracle, weighted, dtype):
         want = run(oracle, "countpairs", "oracle_", options=_opts(**o), **kw)
         assert_same("countpairs", outs[k], want, what="catalog %d" % k)
         assert np.array_equal(outs[k]["npairs"], single["npairs"])
+
+
+@pytest.mark.parametrize("nbytes", [1, 15, 16, 4096 + 7, (1 << 22) + 3])
+def test_device_copy(engine, nbytes):
+    """Device-to-device copies (the engine's copy kernel, with its byte tail) are exact."""
+    src = np.random.default_rng(nbytes).integers(0, 256, nbytes, dtype=np.uint8)
+    a = _lib.DeviceArray(src)
+    b = _lib.DeviceArray(np.zeros(nbytes, np.uint8))
+    b.copy_from(a)
+    assert np.array_equal(b.numpy(), src)
+    a.free()
+    b.free()
