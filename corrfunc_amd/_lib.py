@@ -16,7 +16,7 @@ class cf_amd_stats(C.Structure):
     _fields_ = [("nmesh", C.c_int * 3), ("refine", C.c_int * 3), ("num_cell_pairs", C.c_int64),
                 ("num_work_items", C.c_int64), ("tested_pairs", C.c_double), ("kernel_ms", C.c_double),
                 ("grid_ms", C.c_double), ("total_ms", C.c_double), ("engine", C.c_int), ("sched", C.c_int),
-                ("list_ms", C.c_double)]
+                ("list_ms", C.c_double), ("hist_copies", C.c_int), ("autotuned", C.c_int)]
 
 
 ENGINE_TUNED = 0
@@ -63,6 +63,9 @@ def lib():
                                               C.POINTER(_abi.config_options), C.POINTER(_abi.extra_options)]
         L.cf_amd_countpairs_batch.restype = C.c_int
         for name, args, res in [
+                ("cf_amd_set_autotune", [C.c_int], C.c_int),
+                ("cf_amd_get_autotune", [], C.c_int),
+                ("cf_amd_autotune_clear", [], C.c_int),
                 ("cf_amd_set_scheduler", [C.c_int], C.c_int),
                 ("cf_amd_get_scheduler", [], C.c_int),
                 ("cf_amd_record_costs", [C.c_int], C.c_int),
@@ -96,7 +99,20 @@ def last_stats():
     lib().cf_amd_last_stats(C.byref(s))
     return dict(nmesh=tuple(s.nmesh), refine=tuple(s.refine), num_cell_pairs=s.num_cell_pairs,
                 num_work_items=s.num_work_items, tested_pairs=s.tested_pairs, kernel_ms=s.kernel_ms,
-                grid_ms=s.grid_ms, total_ms=s.total_ms, engine=s.engine, sched=s.sched, list_ms=s.list_ms)
+                grid_ms=s.grid_ms, total_ms=s.total_ms, engine=s.engine, sched=s.sched, list_ms=s.list_ms,
+                hist_copies=s.hist_copies, autotuned=s.autotuned)
+
+
+def set_autotune(on=True):
+    lib().cf_amd_set_autotune(1 if on else 0)
+
+
+def get_autotune():
+    return bool(lib().cf_amd_get_autotune())
+
+
+def autotune_clear():
+    lib().cf_amd_autotune_clear()
 
 
 class DeviceArray:

@@ -16,6 +16,7 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -91,6 +92,13 @@ struct Engine {
     double ns_per_tick = 10.0;  // device wall clock period
     CostModel model;
     DevLattice hlats[2];
+    // autotuner
+    bool autotune = false;
+    bool tuning = false;            // inside a candidate run
+    int force_R = 0;                // histogram replication override (0: automatic)
+    int last_autotuned = 0;         // 0 off, 1 cached choice, 2 tuned by this call
+    bool tune_loaded = false;       // CORRFUNC_AMD_TUNE_CACHE read
+    std::unordered_map<std::string, std::array<int, 4>> tune_cache;  // refine x, y, z, R
 };
 
 static Engine& eng()
@@ -114,6 +122,7 @@ static Engine& eng()
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E.device) == hipSuccess && khz > 0)
             E.ns_per_tick = 1e6 / khz;
+        if (const char* at = getenv("CORRFUNC_AMD_AUTOTUNE")) E.autotune = atoi(at) != 0;
         memset(&E.last, 0, sizeof E.last);
         E.init = true;
     }
@@ -639,6 +648,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         const int v = atoi(rc);
         if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
     }
+    if (E.force_R > 0 && E.force_R < R) R = E.force_R;
     P.hist_copies = R;
     const size_t lds = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
                        (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueue * 5;
@@ -744,6 +754,116 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     T.nitems = nitems;
     T.ticks = ticks;
     T.cu = cu;
+    return EXIT_SUCCESS;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Signature of a workload for the autotuner's cache: mode, the octave of each
+// set's size, rmax and pimax relative to the box (1/16 octave), the box's
+// periodic axes, weights, averages and input precision.
+static std::string tune_key(int mode, int64_t N1, int64_t N2, int autocorr, double rmax, double pimax,
+                            const double ext[6], const double wrap[3], int need_w, int need_avg, size_t esize,
+                            int nbin)
+{
+    double span = 0;
+    for (int d = 0; d < 3; d++) span = fmax(span, wrap[d] > 0 ? wrap[d] : ext[3 + d] - ext[d]);
+    auto q = [&](double v) { return v > 0 && span > 0 ? (int)lround(16.0 * log2(v / span)) : -999; };
+    char buf[256];
+    snprintf(buf, sizeof buf, "m%d n%d n%d a%d r%d p%d w%d%d%d W%d A%d e%d b%d", mode,
+             N1 > 0 ? (int)floor(log2((double)N1)) : -1, N2 > 0 ? (int)floor(log2((double)N2)) : -1, autocorr,
+             q(rmax), q(pimax), wrap[0] > 0, wrap[1] > 0, wrap[2] > 0, need_w, need_avg, (int)esize, nbin);
+    return buf;
+}
+
+static void tune_cache_load()
+{
+    Engine& E = eng();
+    const char* path = getenv("CORRFUNC_AMD_TUNE_CACHE");
+    if (!path) return;
+    FILE* fp = fopen(path, "r");
+    if (!fp) return;
+    char line[512];
+    while (fgets(line, sizeof line, fp)) {
+        char* bar = strchr(line, '|');
+        if (!bar) continue;
+        *bar = '\0';
+        std::array<int, 4> v;
+        if (sscanf(bar + 1, "%d %d %d %d", &v[0], &v[1], &v[2], &v[3]) == 4) E.tune_cache[line] = v;
+    }
+    fclose(fp);
+}
+
+static void tune_cache_store(const std::string& key, const std::array<int, 4>& v)
+{
+    const char* path = getenv("CORRFUNC_AMD_TUNE_CACHE");
+    if (!path) return;
+    FILE* fp = fopen(path, "a");
+    if (!fp) return;
+    fprintf(fp, "%s|%d %d %d %d\n", key.c_str(), v[0], v[1], v[2], v[3]);
+    fclose(fp);
+}
+
+static int run_core(const CallSpec& c, struct config_options* o, struct extra_options* extra, CallOut& R);
+
+// Candidate bin refine factors (x, y, z) and histogram replications the
+// autotuner times; the fastest complete call wins.
+static const int kTuneRefine[][3] = {{1, 1, 1}, {2, 2, 1}, {2, 2, 2}, {3, 3, 1}, {3, 3, 2}, {4, 4, 2}};
+static const int kTuneR[] = {0, 8, 2};
+
+// Looks up (or measures) the refine factors and histogram replication of a
+// workload. Measuring runs the call once per candidate.
+static int autotune_choice(const CallSpec& c, const struct config_options* o, struct extra_options* extra,
+                           const std::string& key, std::array<int, 4>& out)
+{
+    Engine& E = eng();
+    if (!E.tune_loaded) {
+        tune_cache_load();
+        E.tune_loaded = true;
+    }
+    auto it = E.tune_cache.find(key);
+    if (it != E.tune_cache.end()) {
+        out = it->second;
+        E.last_autotuned = std::max(E.last_autotuned, 1);
+        return EXIT_SUCCESS;
+    }
+    E.tuning = true;
+    double best = 1e300;
+    std::array<int, 4> pick = {2, 2, 1, 0};
+    for (const auto& f : kTuneRefine) {
+        struct config_options oc = *o;
+        set_custom_bin_refine_factors(&oc, f);
+        E.force_R = 0;
+        CallOut tmp;
+        const double t0 = now_s();
+        const int st = run_core(c, &oc, extra, tmp);
+        const double dt = now_s() - t0;
+        if (st == EXIT_SUCCESS && dt < best) {
+            best = dt;
+            pick = {f[0], f[1], f[2], 0};
+        }
+    }
+    if (E.engine_kind != CF_AMD_ENGINE_PLAIN) {
+        for (int r : kTuneR) {
+            if (r == 0) continue;
+            struct config_options oc = *o;
+            set_custom_bin_refine_factors(&oc, pick.data());
+            E.force_R = r;
+            CallOut tmp;
+            const double t0 = now_s();
+            const int st = run_core(c, &oc, extra, tmp);
+            const double dt = now_s() - t0;
+            if (st == EXIT_SUCCESS && dt < best) {
+                best = dt;
+                pick[3] = r;
+            }
+        }
+    }
+    E.force_R = 0;
+    E.tuning = false;
+    E.tune_cache[key] = pick;
+    tune_cache_store(key, pick);
+    out = pick;
+    E.last_autotuned = 2;
     return EXIT_SUCCESS;
 }
 
@@ -860,6 +980,20 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         wrap[2] = per[2] ? (bz > 0 ? bz : (ext[5] - ext[2])) : 0.;
     }
 
+    // ---- autotuned refine factors / histogram replication
+    if (!E.tuning) {
+        E.force_R = 0;
+        E.last_autotuned = 0;
+        if (E.autotune && get_bin_refine_scheme(o) == BINNING_DFL) {
+            std::array<int, 4> ch;
+            const std::string key = tune_key(mode, c.N1, c.autocorr ? 0 : c.N2, c.autocorr, rmax, pimax, ext, wrap,
+                                             need_w, o->need_avg_sep, esize, nbin);
+            if (autotune_choice(c, o, extra, key, ch) != EXIT_SUCCESS) return EXIT_FAILURE;
+            set_custom_bin_refine_factors(o, ch.data());
+            E.force_R = ch[3];
+        }
+    }
+
     // ---- grid
     int refine[3];
     GridSpec g;
@@ -966,6 +1100,8 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     CF_CHECK(hipEventElapsedTime(&lms, E.ev_list, E.ev0));
 
     st.list_ms = lms;
+    st.hist_copies = tuned ? P.hist_copies : 0;
+    st.autotuned = E.last_autotuned;
     st.sched = tuned ? (E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC
                                                                                   : E.sched_kind)
                      : CF_AMD_SCHED_NONE;
@@ -994,6 +1130,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     const double t_start = now_s();
     const size_t esize = options->float_type;
     const int need_w = extra->weight_method != NONE;
+    E.last_autotuned = 0;
     if (need_w && extra->weight_method != PAIR_PRODUCT) {
         fprintf(stderr, "Error: unknown weight method %d\n", (int)extra->weight_method);
         return EXIT_FAILURE;
@@ -1048,6 +1185,17 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         } else if (wrap[0] != wrap0[0] || wrap[1] != wrap0[1] || wrap[2] != wrap0[2]) {
             fprintf(stderr, "Error: the catalogs of a batch must share their periodic wraps\n");
             return EXIT_FAILURE;
+        }
+        if (E.autotune && get_bin_refine_scheme(&oc) == BINNING_DFL && ND[k] > 0) {
+            // per size class: tuned on a single-catalog call of this catalog
+            std::array<int, 4> ch;
+            const std::string key = tune_key(MODE_DD, ND[k], 0, 1, rmax, rmax, ext, wrap, need_w, oc.need_avg_sep,
+                                             esize, nbin);
+            CallSpec cs{MODE_DD, ND[k], 0, X[k], Y[k], Z[k], nullptr, nullptr, nullptr, 1, binfile, 0, 0, 0, 0, 1};
+            struct extra_options ek = *extra;
+            ek.weights0.weights[0] = need_w ? W[k] : nullptr;
+            if (autotune_choice(cs, &oc, &ek, key, ch) != EXIT_SUCCESS) return EXIT_FAILURE;
+            set_custom_bin_refine_factors(&oc, ch.data());
         }
         int refine[3];
         GridSpec g;
@@ -1156,6 +1304,8 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     st.total_ms = (now_s() - t_start) * 1e3;
     st.engine = CF_AMD_ENGINE_TUNED;
     st.list_ms = lms;
+    st.hist_copies = P.hist_copies;
+    st.autotuned = E.last_autotuned;
     st.sched = E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC : E.sched_kind;
     E.last = st;
     return EXIT_SUCCESS;
@@ -1759,6 +1909,25 @@ int cf_amd_countpairs_batch(int ncat, const int64_t* ND, void** X, void** Y, voi
         }
         reset_bin_refine_factors(options);
         if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_set_autotune(int on)
+{
+    return guarded([&]() -> int {
+        eng().autotune = on != 0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_get_autotune(void) { return guarded([&]() -> int { return eng().autotune ? 1 : 0; }); }
+
+int cf_amd_autotune_clear(void)
+{
+    return guarded([&]() -> int {
+        eng().tune_cache.clear();
+        eng().tune_loaded = false;  // re-read CORRFUNC_AMD_TUNE_CACHE (if set) at the next tuned call
         return EXIT_SUCCESS;
     });
 }

@@ -41,7 +41,25 @@ typedef struct {
     int engine;              /* CF_AMD_ENGINE_* used */
     int sched;               /* CF_AMD_SCHED_* that ordered the work list */
     double list_ms;          /* cost prediction, ordering and work-list construction (device events) */
+    int hist_copies;         /* LDS histogram replication of the TUNED kernel */
+    int autotuned;           /* 0: autotuner off, 1: cached choice used, 2: tuned during this call */
 } cf_amd_stats;
+
+/* Autotuner. When on, a call whose options carry the default (non-custom) bin
+ * refine factors uses the refine factors and LDS histogram replication that ran
+ * fastest for its workload signature (mode, size octaves of both sets, rmax and
+ * pimax relative to the box, periodic axes, weights, averages, precision). On
+ * the first call of a signature every candidate (refine (1,1,1), (2,2,1),
+ * (2,2,2), (3,3,1), (3,3,2), (4,4,2); then replication 8 and 2 on the best) is
+ * timed on the call's own inputs, and the winner is cached in-process and, if
+ * the environment variable CORRFUNC_AMD_TUNE_CACHE names a file, appended to
+ * it (read back at the first tuned call of a later process). The batched entry
+ * point tunes per catalog size octave. Counts follow the reference's rules for
+ * the chosen refine factors (the reference's results do not depend on them).
+ * CORRFUNC_AMD_AUTOTUNE=1 turns it on at load time. */
+int cf_amd_set_autotune(int on);
+int cf_amd_get_autotune(void);
+int cf_amd_autotune_clear(void);  /* forgets the in-process choices (the file is read again) */
 
 /* Work-list schedulers of the TUNED kernel. Its persistent waves pull work items
  * (<= 64 primaries of a cell pair, or a slice of their secondaries) in list order.

@@ -175,3 +175,63 @@ def test_wp_cell_timer(engine):
                            pimax=20.0, boxsize=L, options=_abi.make_options(isa="fallback"))
     assert np.array_equal(plain["npairs"], out["npairs"])
     assert o.totncells_timings == 0  # released
+
+
+@pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_xi"])
+def test_autotuner_choice_keeps_counts(engine, oracle, mode, tmp_path):
+    """The autotuner's refine factors give the oracle's counts for those factors
+    (and the default factors' counts); the second call reuses the cached choice,
+    also from the cache file in a fresh in-process cache."""
+    L = 150.0
+    X = clustered(40000, L, 61, nparent=30, sigma=2.0, dtype=np.float64)
+    W = weights(40000, 62, np.float64)
+    bins = np.logspace(np.log10(0.3), np.log10(12.0), 10)
+    kw = mode_kwargs(mode, X, W, autocorr=1, bins=bins, L=L, pimax=10.0)
+    o = dict(periodic=True, boxsize=L, need_avg_sep=True, isa="fallback")
+    cache = str(tmp_path / "tune.txt")
+    os.environ["CORRFUNC_AMD_TUNE_CACHE"] = cache
+    try:
+        _lib.autotune_clear()
+        _lib.set_autotune(True)
+        got = run(engine, mode, options=_abi.make_options(**o), **kw)
+        st = _lib.last_stats()
+        assert st["autotuned"] == 2
+        chosen = st["refine"]
+        got2 = run(engine, mode, options=_abi.make_options(**o), **kw)
+        assert _lib.last_stats()["autotuned"] == 1 and _lib.last_stats()["refine"] == chosen
+        _lib.autotune_clear()
+        got3 = run(engine, mode, options=_abi.make_options(**o), **kw)
+        assert _lib.last_stats()["autotuned"] == 1 and _lib.last_stats()["refine"] == chosen
+        assert len(open(cache).read().strip().splitlines()) == 1
+    finally:
+        _lib.set_autotune(False)
+        _lib.autotune_clear()
+        del os.environ["CORRFUNC_AMD_TUNE_CACHE"]
+    oc = _abi.make_options(**o)
+    for i in range(3):
+        oc.bin_refine_factors[i] = chosen[i]
+    oc.binning_flags = (oc.binning_flags & ~0xF) | 0x1  # BINNING_CUST: exactly these factors
+    want = run(oracle, mode, "oracle_", options=oc, **kw)
+    want_dflt = run(oracle, mode, "oracle_", options=_abi.make_options(**o), **kw)
+    for g in (got, got2, got3):
+        assert_same(mode, g, want, what="refine %s" % (chosen,))
+        assert_same(mode, g, want_dflt, what="default refine")
+
+
+def test_autotuned_batch(engine, oracle):
+    L = 200.0
+    cats = [clustered(n, L, 70 + k, nparent=20, sigma=3.0) for k, n in enumerate([3000, 20000, 9000, 60000])]
+    bins = np.logspace(np.log10(0.5), np.log10(20.0), 12)
+    o = dict(periodic=True, boxsize=L, need_avg_sep=True, isa="avx512f")
+    _lib.autotune_clear()
+    _lib.set_autotune(True)
+    try:
+        outs = _lib.countpairs_batch(cats, bins, options=_abi.make_options(**o))
+        assert _lib.last_stats()["autotuned"] == 2
+    finally:
+        _lib.set_autotune(False)
+        _lib.autotune_clear()
+    for k, c in enumerate(cats):
+        want = run(oracle, "countpairs", "oracle_", options=_abi.make_options(**o), autocorr=1, nthreads=2,
+                   binfile=bins, X1=c[0], Y1=c[1], Z1=c[2])
+        assert_same("countpairs", outs[k], want, what="catalog %d" % k)
