@@ -1,0 +1,62 @@
+"""Pair counts -> correlation functions (host numpy), the immediate callers of
+the pair counters in user workflows.
+
+  convert_3d_counts_to_cf      Corrfunc/utils.py:27-164   (Landy-Szalay xi)
+  convert_rp_pi_counts_to_wp   Corrfunc/utils.py:167-322  (projected w_p)
+
+Same arguments, checks, NaN convention and errors as the reference.
+"""
+import numpy as np
+
+__all__ = ("convert_3d_counts_to_cf", "convert_rp_pi_counts_to_wp")
+
+
+def _npairs(a):
+    """The npairs column of a results array, or the array itself."""
+    a = np.asarray(a)
+    if a.dtype.names is not None and "npairs" in a.dtype.names:
+        return a["npairs"]
+    return a
+
+
+def convert_3d_counts_to_cf(ND1, ND2, NR1, NR2, D1D2, D1R2, D2R1, R1R2, estimator="LS"):
+    """Landy-Szalay correlation function from raw pair counts,
+
+        xi = (fN1 fN2 D1D2 - fN1 D1R2 - fN2 D2R1 + R1R2) / R1R2,
+        fN1 = NR1 / ND1,  fN2 = NR2 / ND2,
+
+    NaN where R1R2 is 0. The counts may be arrays or the structured arrays the
+    theory functions return (their npairs column is used)."""
+    counts = [_npairs(a) for a in (D1D2, D1R2, D2R1, R1R2)]
+    nbins = len(counts[0])
+    if any(len(c) != nbins for c in counts[1:]):
+        raise ValueError("Pair counts must have the same number of elements (same bins)")
+    if not ("LS" in estimator or "Landy" in estimator):
+        raise ValueError("Only the Landy-Szalay estimator is supported. Pass estimator='LS'. (Got estimator = {0})"
+                         .format(estimator))
+    dd, dr, rd, rr = counts
+    f1 = np.float64(NR1) / np.float64(ND1)
+    f2 = np.float64(NR2) / np.float64(ND2)
+    cf = np.full(nbins, np.nan)
+    ok = rr > 0
+    cf[ok] = (f1 * f2 * dd[ok] - f1 * dr[ok] - f2 * rd[ok] + rr[ok]) / rr[ok]
+    return cf
+
+
+def convert_rp_pi_counts_to_wp(ND1, ND2, NR1, NR2, D1D2, D1R2, D2R1, R1R2, nrpbins, pimax, dpi=1.0,
+                               estimator="LS"):
+    """Projected correlation function w_p(rp) = 2 dpi sum_pi xi(rp, pi) from
+    (rp, pi) pair counts laid out rp-major with npibins = len / nrpbins pi bins
+    per rp bin; NaN propagates from any pi bin with R1R2 = 0."""
+    if dpi <= 0.0:
+        raise ValueError("Binsize along the line of sight (dpi) = {0}must be positive".format(dpi))
+    xirppi = convert_3d_counts_to_cf(ND1, ND2, NR1, NR2, D1D2, D1R2, D2R1, R1R2, estimator=estimator)
+    npibins = len(xirppi) // nrpbins
+    if npibins * nrpbins != len(xirppi):
+        raise ValueError("Number of pi bins could not be calculated correctly. Expected to find that the total "
+                         "number of bins = {0} would be the product of the number of pi bins = {1} and the number "
+                         "of rp bins = {2}".format(len(xirppi), npibins, nrpbins))
+    if dpi * npibins != pimax:
+        raise ValueError("Pimax = {0} should be equal to the product of npibins = {1} and dpi = {2}. Check your "
+                         "binning scheme.".format(pimax, npibins, dpi))
+    return 2.0 * dpi * np.array([np.sum(xirppi[i * npibins:(i + 1) * npibins]) for i in range(nrpbins)])
