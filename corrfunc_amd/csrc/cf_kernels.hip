@@ -921,17 +921,15 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
                 L2 = L1;
             }
         }
+        // per-cell-pair cost record: wall-clock ticks of the wave on this item
+        const unsigned long long t0 = cp_ticks ? wall_clock64() : 0ull;
+        process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
         if (cp_ticks) {
-            // per-cell-pair cost record: wall-clock ticks of the wave on this item
-            const unsigned long long t0 = wall_clock64();
-            process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
             const unsigned long long dt = wall_clock64() - t0;
             if (lane == 0) {
                 atomicAdd(&cp_ticks[it.cp], dt);
                 cp_cu[it.cp] = (int)__smid();
             }
-        } else {
-            process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
         }
     }
     if (BATCH) {
