@@ -901,10 +901,14 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     const int nitems = *nitems_p;
     DevLattice L1 = lats[0], L2 = lats[BATCH ? 0 : 1];
     int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
+    // The loop body has no lane-divergent branch around its wave-wide steps:
+    // every lane executes the queue atomic (lane 0 adds 1, the others 0, so it
+    // is one +1 per wave) and the item broadcast reads lane 0's result. A
+    // divergent `if (lane == 0)` here lets the compiler start the next
+    // iteration's broadcast without lane 0 (observed in the generated code).
     for (;;) {
-        int item = 0;
-        if (lane == 0) item = atomicAdd(counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
+        const int fetched = atomicAdd(counter, lane == 0 ? 1 : 0);
+        const int item = __builtin_amdgcn_readlane(fetched, 0);
         if (item >= nitems) break;
         const WorkItem it = items[item];
         const CellPairRec cp = cps[__builtin_amdgcn_readfirstlane(it.cp)];
@@ -922,14 +926,14 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
             }
         }
         // per-cell-pair cost record: wall-clock ticks of the wave on this item
+        // (branch-free in the lanes: lane 0 adds dt, the others 0; every lane
+        // stores the same compute-unit id)
         const unsigned long long t0 = cp_ticks ? wall_clock64() : 0ull;
         process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
         if (cp_ticks) {
             const unsigned long long dt = wall_clock64() - t0;
-            if (lane == 0) {
-                atomicAdd(&cp_ticks[it.cp], dt);
-                cp_cu[it.cp] = (int)__smid();
-            }
+            atomicAdd(&cp_ticks[it.cp], lane == 0 ? dt : 0ull);
+            cp_cu[it.cp] = (int)__smid();
         }
     }
     if (BATCH) {

@@ -135,7 +135,7 @@ def test_learned_cost_model_beats_the_heuristic(engine):
         trainX.append(Xs)
         trainy.append(ys)
     Xt, yt = np.concatenate(trainX), np.concatenate(trainy)
-    loss = scheduler.train(Xt, yt, epochs=40, batch=256, lr=3e-3, seed=0)
+    loss = scheduler.train(Xt, yt, epochs=10, batch=512, lr=3e-3, seed=0)
     held = catalogs.nfw_halos(n=400000, L=L, nhalo=1500, seed=104)
     _, (Xh, yh) = _record_xi(held, bins, L)
     ev = scheduler.evaluate(Xh, yh, offset_from=(Xt, yt), workers=2048)
