@@ -97,9 +97,11 @@ constexpr int kNumFeatures = 10;
 // f7 fraction of the secondary cell's z extent within z reach of the primaries
 // f8 projected (rp, pi) mode                f9 log2(1 + N2 * f7)
 
-// Cost-ordered work lists: cell pairs are bucketed by log2 cost, descending.
+// Cost-ordered work lists: heavy cell pairs are bucketed by log2 cost, descending.
 constexpr float kCostTop = 48.0f;        // log2 cost mapped to bucket 0
-constexpr int kBucketsPerOctave = 16;
-constexpr int kCostBuckets = 48 * 16;
+constexpr int kSortPerOctave = 4;
+constexpr int kSortBuckets = 48 * 4 + 1; // + the light bucket (list order)
+constexpr int kSortBlock = 1024;         // cell pairs per block of the sort
+constexpr float kHeavyFactor = 4.0f;     // heavy: cost >= 4 x the mean cell pair
 
 }  // namespace cfamd

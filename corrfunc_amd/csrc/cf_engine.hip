@@ -651,7 +651,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     if (E.force_R > 0 && E.force_R < R) R = E.force_R;
     P.hist_copies = R;
     const size_t lds = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
-                       (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueue * 5;
+                       (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 5;
     if (lds > 64 * 1024) {
         fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
         return EXIT_FAILURE;
@@ -682,18 +682,18 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         else
             k_cost_heuristic<<<nb, 256, 0, E.stream>>>(ncp, cps, lats, batch, cost);
         CF_CHECK(hipGetLastError());
-        int* bcount = (int*)E.arena.get("s_bcount", sizeof(int) * (kCostBuckets + 1));
-        int* boff = (int*)E.arena.get("s_boff", sizeof(int) * (kCostBuckets + 1));
-        int* cursor = (int*)E.arena.get("s_cursor", sizeof(int) * kCostBuckets);
         total = (double*)E.arena.get("s_total", sizeof(double));
-        k_fill_i32<<<4, 256, 0, E.stream>>>(kCostBuckets, bcount, 0);
-        k_fill_i32<<<4, 256, 0, E.stream>>>(kCostBuckets, cursor, 0);
         k_fill_u64<<<1, 64, 0, E.stream>>>(1, (unsigned long long*)total, 0ull);  // +0.0
-        k_cost_bucket<<<nb, 256, 0, E.stream>>>(ncp, cost, bcount, total);
+        k_cost_total<<<std::min(nb, 1024), 256, 0, E.stream>>>(ncp, cost, total);
+        const int nsb = (ncp + kSortBlock - 1) / kSortBlock;
+        const int64_t nmat = (int64_t)kSortBuckets * nsb;
+        int* bcount = (int*)E.arena.get("s_bcount", sizeof(int) * (size_t)(nmat + 1));
+        int* boff = (int*)E.arena.get("s_boff", sizeof(int) * (size_t)(nmat + 1));
+        k_cost_hist<<<nsb, 256, 0, E.stream>>>(ncp, cost, total, nsb, bcount);
         CF_CHECK(hipGetLastError());
-        device_scan(bcount, boff, kCostBuckets, "s_scan");
+        device_scan(bcount, boff, nmat, "s_scan");
         order = (int*)E.arena.get("s_order", sizeof(int) * ncp);
-        k_cost_scatter<<<nb, 256, 0, E.stream>>>(ncp, cost, boff, cursor, order);
+        k_cost_place<<<nsb, 256, 0, E.stream>>>(ncp, cost, total, nsb, boff, order);
         CF_CHECK(hipGetLastError());
         nparts = (int*)E.arena.get("s_nparts", sizeof(int) * ncp);
     }

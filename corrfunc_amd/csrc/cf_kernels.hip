@@ -564,8 +564,10 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 // Work item: (cell pair, chunk of <= 64 consecutive primary particles).
 // Each wave pulls work items from a device counter until the list is exhausted.
 // Lane l holds primary particle i0+l. The secondary particles of the cell pair,
-// restricted to the z-slabs the chunk can reach, are streamed through the scalar
-// unit (their address is uniform across the wave). Two paths per work item:
+// restricted to the z-slabs the chunk can reach, are streamed in tiles of 64:
+// one coalesced vector load per tile (lane l loads secondary jb+l; the next
+// tile is loaded while the current one is processed), then each secondary is
+// broadcast to all lanes with readlane. Two paths per work item:
 //  * dense (every pair of the cell pair is inside the separation range): each
 //    (i, j) is decided in FP64 in lockstep from a 32-byte (x, y, z, w) record;
 //  * sparse: an FP32 test on cell-local coordinates, with a proven error margin,
@@ -573,11 +575,23 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 //    per-wave LDS queue and decided 64 at a time in FP64.
 // Either way every counted pair is decided by the reference's FP64 arithmetic.
 constexpr int kQueue = 128;
+// Per-wave queue storage: kQueue ring entries plus one scratch entry per lane,
+// which lanes without a survivor write to (the enqueue is branch-free).
+constexpr int kQueueAlloc = kQueue + 64;
 
-// Secondary particles are read through the constant address space so that the
-// wave-uniform loads become scalar (SMEM) loads into SGPRs.
-typedef const __attribute__((address_space(4))) float4* uniform_f4_ptr;
-typedef const __attribute__((address_space(4))) double4* uniform_d4_ptr;
+// Broadcast of lane k's value (k wave-uniform; reads the VGPR whatever the
+// execution mask).
+__device__ __forceinline__ float bcast_f(float v, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ double bcast_d(double v, int k)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // LDS histogram replicated R ways with layout [slot][R]: lanes of one wave that
 // land in the same slot add to different copies, which sit in different banks.
@@ -721,8 +735,6 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
 {
     const int lane = lane_id();
     const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
-    uniform_f4_ptr f4j = (uniform_f4_ptr)L2.f4;
-    uniform_d4_ptr d4j = (uniform_d4_ptr)L2.d4;
     const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
     const int flags = __builtin_amdgcn_readfirstlane(it.flags);
     const int dense = P.dense_mode == 1 ? 1 : (P.dense_mode == 2 ? 0 : (flags & kItemDense));
@@ -773,12 +785,24 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         }
     }
 
+    if (jhi <= jlo) return;
+    const int jlast = jhi - 1;
+
     if (dense) {
         const int same = cp.same;
-        for (int j = jlo; j < jhi; j++) {
-            const double xj = d4j[j].x, yj = d4j[j].y, zj = d4j[j].z, wj = d4j[j].w;
-            exact_pair<MODE, R2F>(P, cp, L1, i, ivalid && (!same || j > i), xpos, ypos, zpos, wi, xj, yj, zj, wj,
-                                  S);
+        const double4* __restrict__ d4 = L2.d4;
+        double4 cur = d4[min(jlo + lane, jlast)];
+        for (int jb = jlo; jb < jhi; jb += 64) {
+            const double4 nxt = d4[min(jb + 64 + lane, jlast)];
+            const int m = min(64, jhi - jb);
+            for (int k = 0; k < m; k++) {
+                const int j = jb + k;
+                const double xj = bcast_d(cur.x, k), yj = bcast_d(cur.y, k), zj = bcast_d(cur.z, k);
+                const double wj = P.need_w ? bcast_d(cur.w, k) : 0.0;
+                exact_pair<MODE, R2F>(P, cp, L1, i, ivalid && (!same || j > i), xpos, ypos, zpos, wi, xj, yj, zj,
+                                      wj, S);
+            }
+            cur = nxt;
         }
         return;
     }
@@ -788,38 +812,47 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     const float xr = (float)(xpos - ox), yr = (float)(ypos - oy), zr_i = (float)(zpos - oz);
     const int istride = stride > 0 ? stride : 1;
     int head = 0, tail = 0, count = 0;
-    for (int j = jlo; j < jhi; j++) {
-        const float pjx = f4j[j].x, pjy = f4j[j].y, pjz = f4j[j].z;
-        const float dx = pjx - xr, dy = pjy - yr, dz = pjz - zr_i;
-        float q;
-        int pass;
-        if (MODE == MODE_RPPI || MODE == MODE_WP) {
-            q = fmaf(dy, dy, dx * dx);
-            pass = ivalid && q < hi2 && q >= lo2 && fabsf(dz) < zr;
-        } else {
-            q = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            pass = ivalid && q < hi2 && q >= lo2;
-        }
-        if (cp.same) pass = pass && (j > i);
-        const unsigned long long m = __ballot(pass);
-        if (m) {
-            if (pass) {
-                const int before = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-                const int pos = (tail + before) & (kQueue - 1);
+    const int same = cp.same;
+    const float4* __restrict__ f4 = L2.f4;
+    float4 cur = f4[min(jlo + lane, jlast)];
+    for (int jb = jlo; jb < jhi; jb += 64) {
+        const float4 nxt = f4[min(jb + 64 + lane, jlast)];
+        const int mt = min(64, jhi - jb);
+        for (int k = 0; k < mt; k++) {
+            const int j = jb + k;
+            const float dx = bcast_f(cur.x, k) - xr, dy = bcast_f(cur.y, k) - yr, dz = bcast_f(cur.z, k) - zr_i;
+            float q;
+            int pass;
+            if (MODE == MODE_RPPI || MODE == MODE_WP) {
+                q = fmaf(dy, dy, dx * dx);
+                pass = ivalid && q < hi2 && q >= lo2 && fabsf(dz) < zr;
+            } else {
+                q = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                pass = ivalid && q < hi2 && q >= lo2;
+            }
+            if (same) pass = pass && (j > i);
+            const unsigned long long m = __ballot(pass);
+            if (m) {
+                // branch-free enqueue: survivors take consecutive ring entries,
+                // the other lanes write their scratch entry
+                const int before =
+                    __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+                const int pos = pass ? ((tail + before) & (kQueue - 1)) : (kQueue + lane);
                 qj[pos] = j;
                 ql[pos] = (unsigned char)lane;
-            }
-            const int added = __popcll(m);
-            tail += added;
-            count += added;
-            if (count >= 64) {
-                __builtin_amdgcn_wave_barrier();
-                drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qj, ql, head, 64, S);
-                __builtin_amdgcn_wave_barrier();
-                head += 64;
-                count -= 64;
+                const int added = __popcll(m);
+                tail += added;
+                count += added;
+                if (count >= 64) {
+                    __builtin_amdgcn_wave_barrier();
+                    drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qj, ql, head, 64, S);
+                    __builtin_amdgcn_wave_barrier();
+                    head += 64;
+                    count -= 64;
+                }
             }
         }
+        cur = nxt;
     }
     if (count > 0) {
         __builtin_amdgcn_wave_barrier();
@@ -888,8 +921,8 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
     S.e = e;
     S.lut = lut;
-    int* qj = qbase + wave * kQueue;
-    unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + nwaves * kQueue) + wave * kQueue;
+    int* qj = qbase + wave * kQueueAlloc;
+    unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + nwaves * kQueueAlloc) + wave * kQueueAlloc;
     for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
     for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
     for (int s = threadIdx.x; s < nreg; s += blockDim.x) {

@@ -6,8 +6,10 @@
 //   2. predicts each cell pair's cost, in log2 units, either with the plain
 //      N1*N2 heuristic (k_cost_heuristic) or with the cost model's network
 //      (k_mlp_predict, cf_model.hip)
-//   3. sorts the cell pairs by descending cost with a counting sort over
-//      kCostBuckets buckets of 1/16 octave        (k_cost_bucket, k_cost_scatter)
+//   3. moves the heavy cell pairs (>= kHeavyFactor x the mean cost) to the front,
+//      heaviest first, with a counting sort over quarter-octave buckets (block-local
+//      LDS histograms, a scan, block-local placement); the rest keep list order
+//                                   (k_cost_total, k_cost_hist, scan, k_cost_place)
 //   4. splits heavy cell pairs into parts of their secondary window
 //      (k_count_chunks / k_emit_chunks with the cost array).
 // The counts do not depend on the order or the split: every (i, j) pair is
@@ -67,34 +69,64 @@ __global__ void k_cost_heuristic(int ncp, const CellPairRec* __restrict__ cps, c
     cost[t] = (float)log2(1.0 + pairs);
 }
 
-__device__ __forceinline__ int cost_bucket(float c)
+// Sort keys: cell pairs at least kHeavyFactor times the mean cost go first,
+// heaviest first (4 buckets per octave); all others keep their list order in
+// one last bucket, which keeps neighbouring cells' pairs together (L2 reuse).
+__device__ __forceinline__ int sort_bucket(float c, float light_below)
 {
     c = sane_cost(c);
-    int b = (int)((kCostTop - c) * (float)kBucketsPerOctave);
-    return b < 0 ? 0 : (b >= kCostBuckets ? kCostBuckets - 1 : b);
+    if (c < light_below) return kSortBuckets - 1;
+    int b = (int)((kCostTop - c) * (float)kSortPerOctave);
+    return b < 0 ? 0 : (b > kSortBuckets - 2 ? kSortBuckets - 2 : b);
 }
 
-// Bucket histogram of the costs and the total cost sum(2^cost) (for the split cap).
-__global__ void k_cost_bucket(int ncp, const float* __restrict__ cost, int* __restrict__ counts,
-                              double* __restrict__ total)
+// total = sum over cell pairs of 2^cost.
+__global__ void k_cost_total(int ncp, const float* __restrict__ cost, double* __restrict__ total)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
-    if (t < ncp) {
-        v = exp2((double)sane_cost(cost[t]));
-        atomicAdd(&counts[cost_bucket(cost[t])], 1);
-    }
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ncp; t += gridDim.x * blockDim.x)
+        v += exp2((double)sane_cost(cost[t]));
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(total, v);
 }
 
-__global__ void k_cost_scatter(int ncp, const float* __restrict__ cost, const int* __restrict__ bucket_off,
-                               int* __restrict__ cursor, int* __restrict__ order)
+__device__ __forceinline__ float light_threshold(int ncp, const double* total)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncp) return;
-    const int b = cost_bucket(cost[t]);
-    order[bucket_off[b] + atomicAdd(&cursor[b], 1)] = t;
+    return (float)log2(fmax(*total / (double)ncp * (double)kHeavyFactor, 1.0));
+}
+
+// Per-block bucket histogram (LDS atomics), written bucket-major:
+// counts[b * nblocks + block]. kSortBlock elements per block.
+__global__ __launch_bounds__(256) void k_cost_hist(int ncp, const float* __restrict__ cost,
+                                                   const double* __restrict__ total, int nblocks,
+                                                   int* __restrict__ counts)
+{
+    __shared__ int h[kSortBuckets];
+    for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const float lt = light_threshold(ncp, total);
+    for (int k = 0; k < kSortBlock / 256; k++) {
+        const int t = blockIdx.x * kSortBlock + k * 256 + threadIdx.x;
+        if (t < ncp) atomicAdd(&h[sort_bucket(cost[t], lt)], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x) counts[(int64_t)b * nblocks + blockIdx.x] = h[b];
+}
+
+// order[pos] = cell pair, pos = offs[b * nblocks + block] + rank within the block.
+__global__ __launch_bounds__(256) void k_cost_place(int ncp, const float* __restrict__ cost,
+                                                    const double* __restrict__ total, int nblocks,
+                                                    const int* __restrict__ offs, int* __restrict__ order)
+{
+    __shared__ int cursor[kSortBuckets];
+    for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x)
+        cursor[b] = offs[(int64_t)b * nblocks + blockIdx.x];
+    __syncthreads();
+    const float lt = light_threshold(ncp, total);
+    for (int k = 0; k < kSortBlock / 256; k++) {
+        const int t = blockIdx.x * kSortBlock + k * 256 + threadIdx.x;
+        if (t < ncp) order[atomicAdd(&cursor[sort_bucket(cost[t], lt)], 1)] = t;
+    }
 }
 
 // Integer and FP64 fills (graph-capturable, unlike memset nodes).
