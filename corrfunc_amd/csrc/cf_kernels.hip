@@ -575,6 +575,7 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 //    per-wave LDS queue and decided 64 at a time in FP64.
 // Either way every counted pair is decided by the reference's FP64 arithmetic.
 constexpr int kQueue = 128;
+constexpr int kFetch = 16;  // work items per queue atomic (see k_pairs_tuned)
 // Per-wave queue storage: kQueue ring entries plus one scratch entry per lane,
 // which lanes without a survivor write to (the enqueue is branch-free).
 constexpr int kQueueAlloc = kQueue + 64;
@@ -934,15 +935,25 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     const int nitems = *nitems_p;
     DevLattice L1 = lats[0], L2 = lats[BATCH ? 0 : 1];
     int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
+    // Items are taken from the queue in batches: kFetch per atomic while more
+    // than kFetch items per wave of the grid remain (as last seen), then one at
+    // a time, so the single counter is not the bottleneck when items are small.
     // The loop body has no lane-divergent branch around its wave-wide steps:
-    // every lane executes the queue atomic (lane 0 adds 1, the others 0, so it
-    // is one +1 per wave) and the item broadcast reads lane 0's result. A
-    // divergent `if (lane == 0)` here lets the compiler start the next
-    // iteration's broadcast without lane 0 (observed in the generated code).
+    // every lane executes the queue atomic (lane 0 adds the batch, the others
+    // 0) and the broadcast reads lane 0's result. A divergent `if (lane == 0)`
+    // here lets the compiler start the next iteration's broadcast without
+    // lane 0 (observed in the generated code).
+    const int tail_zone = (int)min((long)nitems, (long)gridDim.x * nwaves * kFetch * 2);
+    int next = 0, end = 0;
     for (;;) {
-        const int fetched = atomicAdd(counter, lane == 0 ? 1 : 0);
-        const int item = __builtin_amdgcn_readlane(fetched, 0);
-        if (item >= nitems) break;
+        if (next >= end) {
+            const int want = next < nitems - tail_zone ? kFetch : 1;
+            const int fetched = atomicAdd(counter, lane == 0 ? want : 0);
+            next = __builtin_amdgcn_readlane(fetched, 0);
+            end = min(next + want, nitems);
+            if (next >= nitems) break;
+        }
+        const int item = next++;
         const WorkItem it = items[item];
         const CellPairRec cp = cps[__builtin_amdgcn_readfirstlane(it.cp)];
         if (BATCH) {
