@@ -401,6 +401,9 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the workload's point count (smaller test runs)")
     ap.add_argument("--sched", default="learned", choices=["none", "heuristic", "learned"])
     ap.add_argument("--sched-ab", action="store_true", help="also time the pair kernel under every scheduler")
+    ap.add_argument("--sweep", default="", metavar="VAR=v1,v2,...",
+                    help="also time the tuned step with the engine variable VAR (read on every call, e.g. "
+                         "CORRFUNC_AMD_DENSE_FRAC) set to each value")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times with the engine's copy kernel "
                          "(the known-size kernel that scales the HBM byte counters)")
@@ -481,6 +484,29 @@ def main():
             ab[name] = {"kernel_ms": float(np.median(ms)), "sched_used": int(st[0]["sched"])}
         scheduler.set_scheduler(cur)
 
+    sweep = None
+    if args.sweep:
+        var, vals = args.sweep.split("=", 1)
+        old = os.environ.get(var)
+        sweep = {}
+        for v in vals.split(","):
+            os.environ[var] = v
+            wl.step()
+            ms, ks = [], []
+            for _ in range(max(1, args.steps)):
+                t0 = time.perf_counter()
+                outs, st = wl.step()
+                ms.append((time.perf_counter() - t0) * 1e3)
+                ks.append(st[0]["kernel_ms"])
+            same = all(np.array_equal(a["npairs"], b["npairs"]) for a, b in zip(outs, out_t))
+            sweep[v] = {"ms_per_step": float(np.median(ms)), "kernel_ms": float(np.median(ks)), "npairs_equal": same}
+            print("sweep %s=%s: %.2f ms/step, kernel %.3f ms" % (var, v, sweep[v]["ms_per_step"],
+                                                                sweep[v]["kernel_ms"]), file=sys.stderr, flush=True)
+        if old is None:
+            del os.environ[var]
+        else:
+            os.environ[var] = old
+
     cpu = parity_cpu = None
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, 16)
@@ -518,6 +544,8 @@ def main():
     }
     if ab is not None:
         line["scheduler_ab"] = ab
+    if sweep is not None:
+        line["sweep"] = {args.sweep.split("=", 1)[0]: sweep}
     if args.workload == 4 and hasattr(wl, "train_loss"):
         line["cost_model"] = {"train_samples": wl.train_n, "train_loss": wl.train_loss, "train_s": wl.train_s}
     print(json.dumps(line))

@@ -74,7 +74,7 @@ struct PairParams {
     int dense_mode;          // 0: by work-item flag, 1: always dense, 2: never dense
     int hist_copies;         // LDS histogram replication R (power of 2): lane l adds to copy l % R
     int dense_stride;        // 1: dense work items take strided primaries
-    int pad_i2;
+    float dense_frac;        // a cell pair whose estimated in-range fraction reaches this takes the dense path
 };
 
 // Work item of the tuned kernel: (cell pair, <= 64 primary particles, one part

@@ -590,6 +590,8 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         P.dense_mode = dm ? atoi(dm) : 0;
         const char* ds = getenv("CORRFUNC_AMD_STRIDE");
         P.dense_stride = ds ? atoi(ds) : 1;
+        const char* df = getenv("CORRFUNC_AMD_DENSE_FRAC");
+        P.dense_frac = df ? (float)atof(df) : 0.35f;
     }
 
 }
@@ -673,7 +675,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         CF_CHECK(hipGetLastError());
     }
     float* cost = nullptr;
-    int *order = nullptr, *nparts = nullptr;
+    int *order = nullptr, *nparts = nullptr, *nheavy = nullptr;
     double* total = nullptr;
     if (sched != CF_AMD_SCHED_NONE) {
         cost = (float*)E.arena.get("s_cost", sizeof(float) * ncp);
@@ -695,6 +697,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         order = (int*)E.arena.get("s_order", sizeof(int) * ncp);
         k_cost_place<<<nsb, 256, 0, E.stream>>>(ncp, cost, total, nsb, boff, order);
         CF_CHECK(hipGetLastError());
+        nheavy = boff + (int64_t)(kSortBuckets - 1) * nsb;  // cell pairs ahead of the light bucket
         nparts = (int*)E.arena.get("s_nparts", sizeof(int) * ncp);
     }
 
@@ -714,6 +717,13 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
     k_fill_i32<<<1, 64, 0, E.stream>>>(1, counter, 0);
     const int* d_nitems = off + ncp;
+    // items taken one at a time: the heavy cell pairs' (all of them without an order)
+    int* single = (int*)E.arena.get("wi_single", sizeof(int));
+    if (nheavy)
+        k_single_boundary<<<1, 64, 0, E.stream>>>(nheavy, off, single);
+    else
+        k_fill_i32<<<1, 64, 0, E.stream>>>(1, single, INT32_MAX);
+    CF_CHECK(hipGetLastError());
     long blocks = max_blocks;
     const long needed = ((long)nitems + nwaves - 1) / nwaves;
     if (blocks > needed) blocks = needed > 0 ? needed : 1;
@@ -740,7 +750,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
                 batch ? " (batched)" : "");
         return EXIT_FAILURE;
     }
-    void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w, &ticks, &cu};
+    void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w, &ticks, &cu, &single};
     CF_CHECK(hipEventRecord(E.ev0, E.stream));
     CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
     CF_CHECK(hipGetLastError());

@@ -899,7 +899,7 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
                                                      unsigned long long* __restrict__ g_np,
                                                      double* __restrict__ g_avg, double* __restrict__ g_w,
                                                      unsigned long long* __restrict__ cp_ticks,
-                                                     int* __restrict__ cp_cu)
+                                                     int* __restrict__ cp_cu, const int* __restrict__ single_p)
 {
     extern __shared__ unsigned long long smem_u64[];
     KernelLds S;
@@ -935,19 +935,22 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     const int nitems = *nitems_p;
     DevLattice L1 = lats[0], L2 = lats[BATCH ? 0 : 1];
     int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
-    // Items are taken from the queue in batches: kFetch per atomic while more
-    // than kFetch items per wave of the grid remain (as last seen), then one at
-    // a time, so the single counter is not the bottleneck when items are small.
+    // Items below *single_p (the heavy cell pairs the scheduler puts first) are
+    // taken one at a time; the others in batches of kFetch per atomic while
+    // more than 2 kFetch items per wave of the grid remain (as last seen), then
+    // one at a time, so the single counter is not the bottleneck when items
+    // are small.
     // The loop body has no lane-divergent branch around its wave-wide steps:
     // every lane executes the queue atomic (lane 0 adds the batch, the others
     // 0) and the broadcast reads lane 0's result. A divergent `if (lane == 0)`
     // here lets the compiler start the next iteration's broadcast without
     // lane 0 (observed in the generated code).
     const int tail_zone = (int)min((long)nitems, (long)gridDim.x * nwaves * kFetch * 2);
+    const int n_single = *single_p;
     int next = 0, end = 0;
     for (;;) {
         if (next >= end) {
-            const int want = next < nitems - tail_zone ? kFetch : 1;
+            const int want = (next >= n_single && next < nitems - tail_zone) ? kFetch : 1;
             const int fetched = atomicAdd(counter, lane == 0 ? want : 0);
             next = __builtin_amdgcn_readlane(fetched, 0);
             end = min(next + want, nitems);
@@ -993,47 +996,50 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
 
 template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_XI, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_XI, R2_X_SUM_YZ, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_XI, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_RPPI, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_RPPI, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_WP, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_WP, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_SMU, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_SMU, R2_FMA_DX_DY_DZ, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*);
+    double*, unsigned long long*, int*, const int*);
 
 // Largest squared separation between the bounding boxes of a cell pair
-// (primary box shifted by the wrap), per the geometry of the mode; the pair is
-// dense when every pair of its particles is inside the range.
+// (primary box shifted by the wrap), per the geometry of the mode, and whether
+// the pair takes the dense path: every pair of its particles is inside the range,
+// or the estimated in-range fraction -- the share of the squared-separation span
+// [min_sep^2, max_sep^2] below rmax^2 (times the |dz| share below pimax in the
+// projected modes) -- reaches P.dense_frac.
 __device__ __forceinline__ int cp_dense(const PairParams& P, const CellPairRec& cp, const double* b1,
                                         const double* b2, double* maxsep2)
 {
@@ -1041,12 +1047,15 @@ __device__ __forceinline__ int cp_dense(const PairParams& P, const CellPairRec& 
     const double mx = fmax(fabs(b2[1] - (b1[0] + ox)), fabs((b1[1] + ox) - b2[0]));
     const double my = fmax(fabs(b2[3] - (b1[2] + oy)), fabs((b1[3] + oy) - b2[2]));
     const double mz = fmax(fabs(b2[5] - (b1[4] + oz)), fabs((b1[5] + oz) - b2[4]));
-    if (P.mode == MODE_RPPI || P.mode == MODE_WP) {
-        *maxsep2 = mx * mx + my * my;
-        return mx * mx + my * my < 0.98 * P.sqr_rmax && mz < 0.98 * P.pimax;
-    }
-    *maxsep2 = mx * mx + my * my + mz * mz;
-    return mx * mx + my * my + mz * mz < 0.98 * P.sqr_rmax;
+    const bool is2d = P.mode == MODE_RPPI || P.mode == MODE_WP;
+    const double hi = is2d ? mx * mx + my * my : mx * mx + my * my + mz * mz;
+    *maxsep2 = hi;
+    if (is2d ? (hi < 0.98 * P.sqr_rmax && mz < 0.98 * P.pimax) : hi < 0.98 * P.sqr_rmax) return 1;
+    const double lo = cp.mdx * cp.mdx + cp.mdy * cp.mdy + (is2d ? 0.0 : cp.mdz * cp.mdz);
+    double f = hi > lo ? (P.sqr_rmax - lo) / (hi - lo) : 0.0;
+    f = fmin(1.0, fmax(0.0, f));
+    if (is2d) f *= mz > cp.mdz ? fmin(1.0, fmax(0.0, (P.pimax - cp.mdz) / (mz - cp.mdz))) : 1.0;
+    return f >= (double)P.dense_frac;
 }
 
 // A predicted log2 cost clamped to [0, kCostTop] (NaN -> 0), so that any model
@@ -1118,6 +1127,14 @@ __global__ void k_emit_chunks(int ncp, const int* __restrict__ order, const Cell
             items[o++] = w;
         }
     }
+}
+
+// *single = item index where the light cell pairs start: the work-list offset
+// of ordered position *nheavy.
+__global__ void k_single_boundary(const int* __restrict__ nheavy, const int* __restrict__ off,
+                                  int* __restrict__ single)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *single = off[*nheavy];
 }
 
 }  // namespace cfamd
