@@ -75,6 +75,8 @@ struct PairParams {
     int hist_copies;         // LDS histogram replication R (power of 2): lane l adds to copy l % R
     int dense_stride;        // 1: dense work items take strided primaries
     float dense_frac;        // a cell pair whose estimated in-range fraction reaches this takes the dense path
+    int fetch;               // work items per queue atomic outside the single-fetch ranges
+    int pad_i3;
 };
 
 // Work item of the tuned kernel: (cell pair, <= 64 primary particles, one part
@@ -102,6 +104,6 @@ constexpr float kCostTop = 48.0f;        // log2 cost mapped to bucket 0
 constexpr int kSortPerOctave = 4;
 constexpr int kSortBuckets = 48 * 4 + 1; // + the light bucket (list order)
 constexpr int kSortBlock = 1024;         // cell pairs per block of the sort
-constexpr float kHeavyFactor = 4.0f;     // heavy: cost >= 4 x the mean cell pair
+constexpr float kHeavyFactor = 4.0f;     // default heavy threshold: cost >= 4 x the mean cell pair
 
 }  // namespace cfamd

@@ -592,6 +592,8 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         P.dense_stride = ds ? atoi(ds) : 1;
         const char* df = getenv("CORRFUNC_AMD_DENSE_FRAC");
         P.dense_frac = df ? (float)atof(df) : 0.35f;
+        const char* fe = getenv("CORRFUNC_AMD_FETCH");
+        P.fetch = fe ? std::max(1, atoi(fe)) : 16;
     }
 
 }
@@ -691,11 +693,13 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         const int64_t nmat = (int64_t)kSortBuckets * nsb;
         int* bcount = (int*)E.arena.get("s_bcount", sizeof(int) * (size_t)(nmat + 1));
         int* boff = (int*)E.arena.get("s_boff", sizeof(int) * (size_t)(nmat + 1));
-        k_cost_hist<<<nsb, 256, 0, E.stream>>>(ncp, cost, total, nsb, bcount);
+        const char* hv = getenv("CORRFUNC_AMD_HEAVY");
+        const float heavy = hv ? (float)atof(hv) : kHeavyFactor;
+        k_cost_hist<<<nsb, 256, 0, E.stream>>>(ncp, cost, total, heavy, nsb, bcount);
         CF_CHECK(hipGetLastError());
         device_scan(bcount, boff, nmat, "s_scan");
         order = (int*)E.arena.get("s_order", sizeof(int) * ncp);
-        k_cost_place<<<nsb, 256, 0, E.stream>>>(ncp, cost, total, nsb, boff, order);
+        k_cost_place<<<nsb, 256, 0, E.stream>>>(ncp, cost, total, heavy, nsb, boff, order);
         CF_CHECK(hipGetLastError());
         nheavy = boff + (int64_t)(kSortBuckets - 1) * nsb;  // cell pairs ahead of the light bucket
         nparts = (int*)E.arena.get("s_nparts", sizeof(int) * ncp);
@@ -704,7 +708,10 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     // ---- work items
     int* nch = (int*)E.arena.get("wi_n", sizeof(int) * (ncp + 1));
     int* off = (int*)E.arena.get("wi_off", sizeof(int) * (ncp + 1));
-    const double cap_div = (double)max_blocks * nwaves * kSplitPerWave;
+    const char* sp = getenv("CORRFUNC_AMD_SPLIT");
+    const double split = sp ? atof(sp) : (double)kSplitPerWave;
+    // split <= 0: no splitting (cap = total)
+    const double cap_div = split > 0 ? (double)max_blocks * nwaves * split : 1.0;
     k_count_chunks<<<nb, 256, 0, E.stream>>>(ncp, order, cps, lats, batch, cost, total, cap_div, nparts, nch);
     CF_CHECK(hipGetLastError());
     device_scan(nch, off, ncp, "wi_scan");

@@ -575,7 +575,6 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 //    per-wave LDS queue and decided 64 at a time in FP64.
 // Either way every counted pair is decided by the reference's FP64 arithmetic.
 constexpr int kQueue = 128;
-constexpr int kFetch = 16;  // work items per queue atomic (see k_pairs_tuned)
 // Per-wave queue storage: kQueue ring entries plus one scratch entry per lane,
 // which lanes without a survivor write to (the enqueue is branch-free).
 constexpr int kQueueAlloc = kQueue + 64;
@@ -936,8 +935,8 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     DevLattice L1 = lats[0], L2 = lats[BATCH ? 0 : 1];
     int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
     // Items below *single_p (the heavy cell pairs the scheduler puts first) are
-    // taken one at a time; the others in batches of kFetch per atomic while
-    // more than 2 kFetch items per wave of the grid remain (as last seen), then
+    // taken one at a time; the others in batches of P.fetch per atomic while
+    // more than 2 P.fetch items per wave of the grid remain (as last seen), then
     // one at a time, so the single counter is not the bottleneck when items
     // are small.
     // The loop body has no lane-divergent branch around its wave-wide steps:
@@ -945,12 +944,12 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     // 0) and the broadcast reads lane 0's result. A divergent `if (lane == 0)`
     // here lets the compiler start the next iteration's broadcast without
     // lane 0 (observed in the generated code).
-    const int tail_zone = (int)min((long)nitems, (long)gridDim.x * nwaves * kFetch * 2);
+    const int tail_zone = (int)min((long)nitems, (long)gridDim.x * nwaves * P.fetch * 2);
     const int n_single = *single_p;
     int next = 0, end = 0;
     for (;;) {
         if (next >= end) {
-            const int want = (next >= n_single && next < nitems - tail_zone) ? kFetch : 1;
+            const int want = (next >= n_single && next < nitems - tail_zone) ? P.fetch : 1;
             const int fetched = atomicAdd(counter, lane == 0 ? want : 0);
             next = __builtin_amdgcn_readlane(fetched, 0);
             end = min(next + want, nitems);

@@ -69,7 +69,7 @@ __global__ void k_cost_heuristic(int ncp, const CellPairRec* __restrict__ cps, c
     cost[t] = (float)log2(1.0 + pairs);
 }
 
-// Sort keys: cell pairs at least kHeavyFactor times the mean cost go first,
+// Sort keys: cell pairs at least `heavy` times the mean cost go first,
 // heaviest first (4 buckets per octave); all others keep their list order in
 // one last bucket, which keeps neighbouring cells' pairs together (L2 reuse).
 __device__ __forceinline__ int sort_bucket(float c, float light_below)
@@ -90,21 +90,21 @@ __global__ void k_cost_total(int ncp, const float* __restrict__ cost, double* __
     if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(total, v);
 }
 
-__device__ __forceinline__ float light_threshold(int ncp, const double* total)
+__device__ __forceinline__ float light_threshold(int ncp, const double* total, float heavy)
 {
-    return (float)log2(fmax(*total / (double)ncp * (double)kHeavyFactor, 1.0));
+    return (float)log2(fmax(*total / (double)ncp * (double)heavy, 1.0));
 }
 
 // Per-block bucket histogram (LDS atomics), written bucket-major:
 // counts[b * nblocks + block]. kSortBlock elements per block.
 __global__ __launch_bounds__(256) void k_cost_hist(int ncp, const float* __restrict__ cost,
-                                                   const double* __restrict__ total, int nblocks,
+                                                   const double* __restrict__ total, float heavy, int nblocks,
                                                    int* __restrict__ counts)
 {
     __shared__ int h[kSortBuckets];
     for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x) h[b] = 0;
     __syncthreads();
-    const float lt = light_threshold(ncp, total);
+    const float lt = light_threshold(ncp, total, heavy);
     for (int k = 0; k < kSortBlock / 256; k++) {
         const int t = blockIdx.x * kSortBlock + k * 256 + threadIdx.x;
         if (t < ncp) atomicAdd(&h[sort_bucket(cost[t], lt)], 1);
@@ -115,14 +115,14 @@ __global__ __launch_bounds__(256) void k_cost_hist(int ncp, const float* __restr
 
 // order[pos] = cell pair, pos = offs[b * nblocks + block] + rank within the block.
 __global__ __launch_bounds__(256) void k_cost_place(int ncp, const float* __restrict__ cost,
-                                                    const double* __restrict__ total, int nblocks,
+                                                    const double* __restrict__ total, float heavy, int nblocks,
                                                     const int* __restrict__ offs, int* __restrict__ order)
 {
     __shared__ int cursor[kSortBuckets];
     for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x)
         cursor[b] = offs[(int64_t)b * nblocks + blockIdx.x];
     __syncthreads();
-    const float lt = light_threshold(ncp, total);
+    const float lt = light_threshold(ncp, total, heavy);
     for (int k = 0; k < kSortBlock / 256; k++) {
         const int t = blockIdx.x * kSortBlock + k * 256 + threadIdx.x;
         if (t < ncp) order[atomicAdd(&cursor[sort_bucket(cost[t], lt)], 1)] = t;
