@@ -593,7 +593,7 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         const char* df = getenv("CORRFUNC_AMD_DENSE_FRAC");
         P.dense_frac = df ? (float)atof(df) : 0.35f;
         const char* fe = getenv("CORRFUNC_AMD_FETCH");
-        P.fetch = fe ? std::max(1, atoi(fe)) : 16;
+        P.fetch = fe ? std::max(1, atoi(fe)) : 1;
     }
 
 }
@@ -721,8 +721,8 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     WorkItem* items = (WorkItem*)E.arena.get("wi_items", sizeof(WorkItem) * ((size_t)nitems + 1));
     k_emit_chunks<<<nb, 256, 0, E.stream>>>(ncp, order, cps, lats, batch, P, off, nparts, items);
     CF_CHECK(hipGetLastError());
-    int* counter = (int*)E.arena.get("wi_counter", sizeof(int));
-    k_fill_i32<<<1, 64, 0, E.stream>>>(1, counter, 0);
+    int* counter = (int*)E.arena.get("wi_counter", sizeof(int) * kQueues * kCounterStride);
+    k_fill_i32<<<1, 256, 0, E.stream>>>(kQueues * kCounterStride, counter, 0);
     const int* d_nitems = off + ncp;
     // items taken one at a time: the heavy cell pairs' (all of them without an order)
     int* single = (int*)E.arena.get("wi_single", sizeof(int));

@@ -575,6 +575,8 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 //    per-wave LDS queue and decided 64 at a time in FP64.
 // Either way every counted pair is decided by the reference's FP64 arithmetic.
 constexpr int kQueue = 128;
+constexpr int kQueues = 8;          // work sub-queues (one counter each)
+constexpr int kCounterStride = 32;  // ints between sub-queue counters (128 B)
 // Per-wave queue storage: kQueue ring entries plus one scratch entry per lane,
 // which lanes without a survivor write to (the enqueue is branch-free).
 constexpr int kQueueAlloc = kQueue + 64;
@@ -934,28 +936,45 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     const int nitems = *nitems_p;
     DevLattice L1 = lats[0], L2 = lats[BATCH ? 0 : 1];
     int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
-    // Items below *single_p (the heavy cell pairs the scheduler puts first) are
-    // taken one at a time; the others in batches of P.fetch per atomic while
-    // more than 2 P.fetch items per wave of the grid remain (as last seen), then
-    // one at a time, so the single counter is not the bottleneck when items
-    // are small.
-    // The loop body has no lane-divergent branch around its wave-wide steps:
-    // every lane executes the queue atomic (lane 0 adds the batch, the others
-    // 0) and the broadcast reads lane 0's result. A divergent `if (lane == 0)`
-    // here lets the compiler start the next iteration's broadcast without
-    // lane 0 (observed in the generated code).
+    // Work queue: item g belongs to sub-queue g % kQueues (the cost order is kept
+    // inside every sub-queue); sub-queue q has its own counter at
+    // counter[q * kCounterStride], so the device-wide atomics spread over
+    // kQueues addresses. A wave starts at sub-queue blockIdx.x % kQueues and moves
+    // on to the next when that one is exhausted, until all are. Items below
+    // *single_p (the heavy cell pairs the scheduler puts first) are taken one at
+    // a time; the others P.fetch per atomic while more than 2 P.fetch items per
+    // wave of the grid remain (as last seen), then one at a time.
+    // The loop has no lane-divergent branch around its wave-wide steps: every
+    // lane executes the queue atomic (lane 0 adds the batch, the others 0) and
+    // the broadcast reads lane 0's result. A divergent `if (lane == 0)` here lets
+    // the compiler start the next iteration's broadcast without lane 0
+    // (observed in the generated code).
     const int tail_zone = (int)min((long)nitems, (long)gridDim.x * nwaves * P.fetch * 2);
     const int n_single = *single_p;
-    int next = 0, end = 0;
+    const int home = blockIdx.x & (kQueues - 1);
+    int qi = 0, next = 0, end = 0, last = 0;
     for (;;) {
         if (next >= end) {
-            const int want = (next >= n_single && next < nitems - tail_zone) ? P.fetch : 1;
-            const int fetched = atomicAdd(counter, lane == 0 ? want : 0);
-            next = __builtin_amdgcn_readlane(fetched, 0);
-            end = min(next + want, nitems);
-            if (next >= nitems) break;
+            bool got = false;
+            while (qi < kQueues) {
+                const int q = (home + qi) & (kQueues - 1);
+                const int cnt = nitems > q ? (nitems - q + kQueues - 1) / kQueues : 0;
+                const int want = (last >= n_single && last < nitems - tail_zone) ? P.fetch : 1;
+                const int fetched = atomicAdd(&counter[q * kCounterStride], lane == 0 ? want : 0);
+                const int k = __builtin_amdgcn_readlane(fetched, 0);
+                if (k < cnt) {
+                    next = q + kQueues * k;
+                    end = q + kQueues * min(k + want, cnt);
+                    got = true;
+                    break;
+                }
+                qi++;
+            }
+            if (!got) break;
         }
-        const int item = next++;
+        const int item = next;
+        next += kQueues;
+        last = item;
         const WorkItem it = items[item];
         const CellPairRec cp = cps[__builtin_amdgcn_readfirstlane(it.cp)];
         if (BATCH) {
