@@ -401,6 +401,9 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the workload's point count (smaller test runs)")
     ap.add_argument("--sched", default="learned", choices=["none", "heuristic", "learned"])
     ap.add_argument("--sched-ab", action="store_true", help="also time the pair kernel under every scheduler")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tuned form with the engine's autotuner (refine factors, LDS replication); the pairs "
+                         "tested stay those of the default options, counted by an untimed default step")
     ap.add_argument("--sweep", default="", metavar="VAR=v1,v2,...",
                     help="also time the tuned step with the engine variable VAR (read on every call, e.g. "
                          "CORRFUNC_AMD_DENSE_FRAC) set to each value")
@@ -443,8 +446,18 @@ def main():
         # device is idle here
         return time.perf_counter() - t0, outs, stats
 
+    canonical = None
+    if args.autotune:
+        # the unit of work is the default options' tested pairs: count them first
+        _lib.set_engine(_lib.ENGINE_TUNED)
+        _lib.set_autotune(False)
+        _, st0 = wl.step()
+        canonical = sum(s["tested_pairs"] for s in st0)
+        _lib.set_autotune(True)
     t_tuned, out_t, st_t = timed(_lib.ENGINE_TUNED, args.steps, args.warmup, wl.step)
-    tested = sum(s["tested_pairs"] for s in st_t[-1])
+    _lib.set_autotune(False)
+    tested = canonical if canonical is not None else sum(s["tested_pairs"] for s in st_t[-1])
+    engine_tested = sum(s["tested_pairs"] for s in st_t[-1])
     value = tested * args.steps / t_tuned
     useful = sum(float(o["npairs"].sum()) for o in out_t) / 2 * args.steps / t_tuned
     print("tuned: %.4g pairs tested/s (%.1f ms/step)" % (value, t_tuned / args.steps * 1e3), file=sys.stderr,
@@ -530,7 +543,9 @@ def main():
         "dtype": wl.dtype,
         "data": wl.data,
         "config": {"workload": wl.description, "tested_pairs_per_step": tested, "parallelism": "single GPU",
-                   "scheduler": args.sched},
+                   "scheduler": args.sched, "autotune": bool(args.autotune),
+                   "engine_tested_pairs_per_step": engine_tested,
+                   "refine": list(st_t[-1][0]["refine"]), "autotuned": st_t[-1][0]["autotuned"]},
         "plain_value": plain,
         "tuned_over_plain": (value / plain) if plain else None,
         "useful_pairs_per_s": useful,

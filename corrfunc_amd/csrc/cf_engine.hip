@@ -96,9 +96,10 @@ struct Engine {
     bool autotune = false;
     bool tuning = false;            // inside a candidate run
     int force_R = 0;                // histogram replication override (0: automatic)
+    int force_fetch = 0;            // queue batch override (0: CORRFUNC_AMD_FETCH or 1)
     int last_autotuned = 0;         // 0 off, 1 cached choice, 2 tuned by this call
     bool tune_loaded = false;       // CORRFUNC_AMD_TUNE_CACHE read
-    std::unordered_map<std::string, std::array<int, 4>> tune_cache;  // refine x, y, z, R
+    std::unordered_map<std::string, std::array<int, 5>> tune_cache;  // refine x, y, z, R, fetch
 };
 
 static Engine& eng()
@@ -202,18 +203,25 @@ static const void* to_device(const std::string& slot, const void* p, int64_t n, 
 }
 
 // ----------------------------------------------------------- helpers
-static void device_minmax(const void* X, const void* Y, const void* Z, int64_t n, size_t esize, double out[6])
+// Data extent (x, y, z min then max) into res[0..5] on the device (no sync).
+static void device_minmax_async(const void* X, const void* Y, const void* Z, int64_t n, size_t esize, double* res)
 {
     Engine& E = eng();
     const int nb = 512;
     double* part = (double*)E.arena.get("mm_part", sizeof(double) * 6 * nb);
-    double* res = (double*)E.arena.get("mm_res", sizeof(double) * 6);
     if (esize == 4)
         k_minmax<float><<<nb, 256, 0, E.stream>>>(n, (const float*)X, (const float*)Y, (const float*)Z, part);
     else
         k_minmax<double><<<nb, 256, 0, E.stream>>>(n, (const double*)X, (const double*)Y, (const double*)Z, part);
     k_minmax_final<<<1, 64, 0, E.stream>>>(nb, part, res);
     CF_CHECK(hipGetLastError());
+}
+
+static void device_minmax(const void* X, const void* Y, const void* Z, int64_t n, size_t esize, double out[6])
+{
+    Engine& E = eng();
+    double* res = (double*)E.arena.get("mm_res", sizeof(double) * 6);
+    device_minmax_async(X, Y, Z, n, esize, res);
     CF_CHECK(hipMemcpyAsync(out, res, sizeof(double) * 6, hipMemcpyDeviceToHost, E.stream));
     CF_CHECK(hipStreamSynchronize(E.stream));
 }
@@ -282,8 +290,10 @@ static int make_grid(GridSpec& g, const double ext[6], const double gsz[3], cons
 }
 
 // Device counting sort of one particle set into the grid (gridlink_impl.c.src:65-436).
-static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n, const void* X, const void* Y,
-                           const void* Z, const void* W, size_t esize, DevLattice& L)
+// Counting-sort gridding launched on the stream without waiting: particles
+// outside the grid are counted into *oob (zeroed by the caller) and skipped.
+static int gridlink_launch(const std::string& tag, const GridSpec& g, int64_t n, const void* X, const void* Y,
+                           const void* Z, const void* W, size_t esize, DevLattice& L, unsigned long long* oob)
 {
     Engine& E = eng();
     L.nx = g.nx; L.ny = g.ny; L.nz = g.nz;
@@ -301,7 +311,6 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
     int* counts = (int*)E.arena.get(tag + "_cnt", sizeof(int) * (nb + 1));
     int* fill = (int*)E.arena.get(tag + "_fill", sizeof(int) * (nb + 1));
     int* bucket = (int*)E.arena.get(tag + "_bucket", sizeof(int) * (n + 1));
-    unsigned long long* oob = (unsigned long long*)E.arena.get(tag + "_oob", 8);
     L.bstart = (int*)E.arena.get(tag + "_bstart", sizeof(int) * (nb + 1));
     L.x = (double*)E.arena.get(tag + "_x", sizeof(double) * n);
     L.y = (double*)E.arena.get(tag + "_y", sizeof(double) * n);
@@ -312,7 +321,6 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
     L.bounds = (double*)E.arena.get(tag + "_bounds", sizeof(double) * 6 * L.ncells);
     CF_CHECK(hipMemsetAsync(counts, 0, sizeof(int) * (nb + 1), E.stream));
     CF_CHECK(hipMemsetAsync(fill, 0, sizeof(int) * (nb + 1), E.stream));
-    CF_CHECK(hipMemsetAsync(oob, 0, 8, E.stream));
     GridArgs ga{g.xmin, g.ymin, g.zmin, g.xinv, g.yinv, g.zinv, g.xbs, g.ybs, g.zbs, g.nx, g.ny, g.nz};
     const unsigned blocks = (unsigned)((n + 255) / 256);
     if (n > 0) {
@@ -323,13 +331,6 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
             k_bucket_count<double><<<blocks, 256, 0, E.stream>>>(n, (const double*)X, (const double*)Y,
                                                                   (const double*)Z, ga, bucket, counts, oob);
         CF_CHECK(hipGetLastError());
-    }
-    unsigned long long h_oob = 0;
-    CF_CHECK(hipMemcpyAsync(&h_oob, oob, 8, hipMemcpyDeviceToHost, E.stream));
-    CF_CHECK(hipStreamSynchronize(E.stream));
-    if (h_oob) {
-        fprintf(stderr, "Error: %llu particles are out of bounds. Check periodic wrapping?\n", h_oob);
-        return EXIT_FAILURE;
     }
     device_scan(counts, L.bstart, nb, tag + "_scan");
     if (n > 0) {
@@ -343,19 +344,54 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
                                                              L.y, L.z, L.w, L.d4, L.f4);
         CF_CHECK(hipGetLastError());
     }
-    k_cell_bounds<<<(unsigned)((L.ncells + 255) / 256), 256, 0, E.stream>>>(L.ncells, L.bstart, L.x, L.y, L.z,
+    k_cell_bounds<<<(unsigned)((L.ncells + 3) / 4), 256, 0, E.stream>>>(L.ncells, L.bstart, L.x, L.y, L.z,
                                                                            L.bounds);
     CF_CHECK(hipGetLastError());
     return EXIT_SUCCESS;
 }
 
-// Device work list of neighbour-cell pairs (gridlink_impl.c.src:439-625).
-static int cellpairs_device(const DevLattice& L1, const DevLattice& L2, const int refine[3], const int per[3],
-                            const double wrap[3], double max3, double max2, double max1, int min_sep_opt,
-                            int autocorr, CellPairRec** out, int* ncp_out, double* tested, int cat = 0)
+static int check_oob(unsigned long long n_oob)
+{
+    if (n_oob) {
+        fprintf(stderr, "Error: %llu particles are out of bounds. Check periodic wrapping?\n", n_oob);
+        return EXIT_FAILURE;
+    }
+    return EXIT_SUCCESS;
+}
+
+static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n, const void* X, const void* Y,
+                           const void* Z, const void* W, size_t esize, DevLattice& L)
 {
     Engine& E = eng();
+    unsigned long long* oob = (unsigned long long*)E.arena.get(tag + "_oob", 8);
+    CF_CHECK(hipMemsetAsync(oob, 0, 8, E.stream));
+    if (gridlink_launch(tag, g, n, X, Y, Z, W, esize, L, oob) != EXIT_SUCCESS) return EXIT_FAILURE;
+    unsigned long long h_oob = 0;
+    CF_CHECK(hipMemcpyAsync(&h_oob, oob, 8, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    return check_oob(h_oob);
+}
+
+// Device work list of neighbour-cell pairs (gridlink_impl.c.src:439-625), in
+// two steps: cellpairs_flag marks the kept (cell, neighbour) candidates and
+// scans them (the count lands at plan.pos[plan.total] on the device);
+// cellpairs_emit writes the records in the reference's order and adds the pairs
+// tested to *dtested.
+struct CpPlan {
     NgbArgs a;
+    int nngb = 0;
+    int64_t total = 0;
+    int* flags = nullptr;
+    int* pos = nullptr;
+    DevLattice L1, L2;
+};
+
+static int cellpairs_flag(const DevLattice& L1, const DevLattice& L2, const int refine[3], const int per[3],
+                          const double wrap[3], double max3, double max2, double max1, int min_sep_opt, int autocorr,
+                          CpPlan& cp)
+{
+    Engine& E = eng();
+    NgbArgs& a = cp.a;
     a.nx = L1.nx; a.ny = L1.ny; a.nz = L1.nz;
     a.rx = refine[0]; a.ry = refine[1]; a.rz = refine[2];
     a.px = per[0]; a.py = per[1]; a.pz = per[2];
@@ -365,28 +401,48 @@ static int cellpairs_device(const DevLattice& L1, const DevLattice& L2, const in
                  (L1.nx < 2 * refine[0] + 1 || L1.ny < 2 * refine[1] + 1 || L1.nz < 2 * refine[2] + 1);
     a.xwrap = wrap[0]; a.ywrap = wrap[1]; a.zwrap = wrap[2];
     a.max3 = max3; a.max2 = max2; a.max1 = max1;
-    const int nngb = (2 * refine[0] + 1) * (2 * refine[1] + 1) * (2 * refine[2] + 1);
-    const int64_t total = L1.ncells * nngb;
-    if (total >= (int64_t)INT32_MAX) {
-        fprintf(stderr, "Error: %lld candidate cell pairs exceed the engine's 32-bit indexing\n", (long long)total);
+    cp.nngb = (2 * refine[0] + 1) * (2 * refine[1] + 1) * (2 * refine[2] + 1);
+    cp.total = L1.ncells * cp.nngb;
+    cp.L1 = L1;
+    cp.L2 = L2;
+    if (cp.total >= (int64_t)INT32_MAX) {
+        fprintf(stderr, "Error: %lld candidate cell pairs exceed the engine's 32-bit indexing\n", (long long)cp.total);
         return EXIT_FAILURE;
     }
-    int* flags = (int*)E.arena.get("cp_flags", sizeof(int) * (total + 1));
-    int* pos = (int*)E.arena.get("cp_pos", sizeof(int) * (total + 1));
-    double* dtested = (double*)E.arena.get("cp_tested", sizeof(double));
-    const unsigned blocks = (unsigned)((total + 255) / 256);
-    k_cellpair_flag<<<blocks, 256, 0, E.stream>>>(a, L1.ncells, nngb, L1.bstart, L2.bstart, L1.bounds, L2.bounds,
-                                                  flags);
+    cp.flags = (int*)E.arena.get("cp_flags", sizeof(int) * (cp.total + 1));
+    cp.pos = (int*)E.arena.get("cp_pos", sizeof(int) * (cp.total + 1));
+    const unsigned blocks = (unsigned)((cp.total + 255) / 256);
+    k_cellpair_flag<<<blocks, 256, 0, E.stream>>>(a, L1.ncells, cp.nngb, L1.bstart, L2.bstart, L1.bounds, L2.bounds,
+                                                  cp.flags);
     CF_CHECK(hipGetLastError());
-    device_scan(flags, pos, total, "cp_scan");
+    device_scan(cp.flags, cp.pos, cp.total, "cp_scan");
+    return EXIT_SUCCESS;
+}
+
+static void cellpairs_emit(const CpPlan& cp, CellPairRec* out, double* dtested, int cat)
+{
+    Engine& E = eng();
+    const unsigned blocks = (unsigned)((cp.total + 255) / 256);
+    k_cellpair_emit<<<blocks, 256, 0, E.stream>>>(cp.a, cp.L1.ncells, cp.nngb, cp.L1.bstart, cp.L2.bstart,
+                                                  cp.L1.bounds, cp.L2.bounds, cp.pos, out, dtested, cat);
+    CF_CHECK(hipGetLastError());
+}
+
+static int cellpairs_device(const DevLattice& L1, const DevLattice& L2, const int refine[3], const int per[3],
+                            const double wrap[3], double max3, double max2, double max1, int min_sep_opt,
+                            int autocorr, CellPairRec** out, int* ncp_out, double* tested, int cat = 0)
+{
+    Engine& E = eng();
+    CpPlan cp;
+    if (cellpairs_flag(L1, L2, refine, per, wrap, max3, max2, max1, min_sep_opt, autocorr, cp) != EXIT_SUCCESS)
+        return EXIT_FAILURE;
     int ncp = 0;
-    CF_CHECK(hipMemcpyAsync(&ncp, pos + total, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(&ncp, cp.pos + cp.total, sizeof(int), hipMemcpyDeviceToHost, E.stream));
     CF_CHECK(hipStreamSynchronize(E.stream));
     CellPairRec* cps = (CellPairRec*)E.arena.get("cp_list", sizeof(CellPairRec) * (ncp + 1));
+    double* dtested = (double*)E.arena.get("cp_tested", sizeof(double));
     CF_CHECK(hipMemsetAsync(dtested, 0, sizeof(double), E.stream));
-    k_cellpair_emit<<<blocks, 256, 0, E.stream>>>(a, L1.ncells, nngb, L1.bstart, L2.bstart, L1.bounds, L2.bounds,
-                                                  pos, cps, dtested, cat);
-    CF_CHECK(hipGetLastError());
+    cellpairs_emit(cp, cps, dtested, cat);
     CF_CHECK(hipMemcpyAsync(tested, dtested, sizeof(double), hipMemcpyDeviceToHost, E.stream));
     CF_CHECK(hipStreamSynchronize(E.stream));
     *out = cps;
@@ -594,6 +650,7 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         P.dense_frac = df ? (float)atof(df) : 0.35f;
         const char* fe = getenv("CORRFUNC_AMD_FETCH");
         P.fetch = fe ? std::max(1, atoi(fe)) : 1;
+        if (eng().force_fetch > 0) P.fetch = eng().force_fetch;
     }
 
 }
@@ -804,19 +861,19 @@ static void tune_cache_load()
         char* bar = strchr(line, '|');
         if (!bar) continue;
         *bar = '\0';
-        std::array<int, 4> v;
-        if (sscanf(bar + 1, "%d %d %d %d", &v[0], &v[1], &v[2], &v[3]) == 4) E.tune_cache[line] = v;
+        std::array<int, 5> v;
+        if (sscanf(bar + 1, "%d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4]) == 5) E.tune_cache[line] = v;
     }
     fclose(fp);
 }
 
-static void tune_cache_store(const std::string& key, const std::array<int, 4>& v)
+static void tune_cache_store(const std::string& key, const std::array<int, 5>& v)
 {
     const char* path = getenv("CORRFUNC_AMD_TUNE_CACHE");
     if (!path) return;
     FILE* fp = fopen(path, "a");
     if (!fp) return;
-    fprintf(fp, "%s|%d %d %d %d\n", key.c_str(), v[0], v[1], v[2], v[3]);
+    fprintf(fp, "%s|%d %d %d %d %d\n", key.c_str(), v[0], v[1], v[2], v[3], v[4]);
     fclose(fp);
 }
 
@@ -826,11 +883,12 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
 // autotuner times; the fastest complete call wins.
 static const int kTuneRefine[][3] = {{1, 1, 1}, {2, 2, 1}, {2, 2, 2}, {3, 3, 1}, {3, 3, 2}, {4, 4, 2}};
 static const int kTuneR[] = {0, 8, 2};
+static const int kTuneFetch[] = {1, 4};
 
 // Looks up (or measures) the refine factors and histogram replication of a
 // workload. Measuring runs the call once per candidate.
 static int autotune_choice(const CallSpec& c, const struct config_options* o, struct extra_options* extra,
-                           const std::string& key, std::array<int, 4>& out)
+                           const std::string& key, std::array<int, 5>& out)
 {
     Engine& E = eng();
     if (!E.tune_loaded) {
@@ -845,18 +903,19 @@ static int autotune_choice(const CallSpec& c, const struct config_options* o, st
     }
     E.tuning = true;
     double best = 1e300;
-    std::array<int, 4> pick = {2, 2, 1, 0};
+    std::array<int, 5> pick = {2, 2, 1, 0, 1};
     for (const auto& f : kTuneRefine) {
         struct config_options oc = *o;
         set_custom_bin_refine_factors(&oc, f);
         E.force_R = 0;
+        E.force_fetch = 1;
         CallOut tmp;
         const double t0 = now_s();
         const int st = run_core(c, &oc, extra, tmp);
         const double dt = now_s() - t0;
         if (st == EXIT_SUCCESS && dt < best) {
             best = dt;
-            pick = {f[0], f[1], f[2], 0};
+            pick = {f[0], f[1], f[2], 0, 1};
         }
     }
     if (E.engine_kind != CF_AMD_ENGINE_PLAIN) {
@@ -874,7 +933,23 @@ static int autotune_choice(const CallSpec& c, const struct config_options* o, st
                 pick[3] = r;
             }
         }
+        for (int fch : kTuneFetch) {
+            if (fch == 1) continue;
+            struct config_options oc = *o;
+            set_custom_bin_refine_factors(&oc, pick.data());
+            E.force_R = pick[3];
+            E.force_fetch = fch;
+            CallOut tmp;
+            const double t0 = now_s();
+            const int st = run_core(c, &oc, extra, tmp);
+            const double dt = now_s() - t0;
+            if (st == EXIT_SUCCESS && dt < best) {
+                best = dt;
+                pick[4] = fch;
+            }
+        }
     }
+    E.force_fetch = 0;
     E.force_R = 0;
     E.tuning = false;
     E.tune_cache[key] = pick;
@@ -1000,14 +1075,16 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     // ---- autotuned refine factors / histogram replication
     if (!E.tuning) {
         E.force_R = 0;
+        E.force_fetch = 0;
         E.last_autotuned = 0;
         if (E.autotune && get_bin_refine_scheme(o) == BINNING_DFL) {
-            std::array<int, 4> ch;
+            std::array<int, 5> ch;
             const std::string key = tune_key(mode, c.N1, c.autocorr ? 0 : c.N2, c.autocorr, rmax, pimax, ext, wrap,
                                              need_w, o->need_avg_sep, esize, nbin);
             if (autotune_choice(c, o, extra, key, ch) != EXIT_SUCCESS) return EXIT_FAILURE;
             set_custom_bin_refine_factors(o, ch.data());
             E.force_R = ch[3];
+            E.force_fetch = ch[4];
         }
     }
 
@@ -1148,6 +1225,10 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     const size_t esize = options->float_type;
     const int need_w = extra->weight_method != NONE;
     E.last_autotuned = 0;
+    E.force_R = 0;
+    E.force_fetch = 0;
+    int batch_fetch = 0;          // the autotuner's queue batch for the largest catalog
+    int64_t batch_fetch_n = -1;
     if (need_w && extra->weight_method != PAIR_PRODUCT) {
         fprintf(stderr, "Error: unknown weight method %d\n", (int)extra->weight_method);
         return EXIT_FAILURE;
@@ -1168,27 +1249,45 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         if (!(rupp_sqr[i] >= rupp_sqr[i - 1])) sorted = 0;
     const int nslots = nbin;
 
+    // Phased over all catalogs, one host synchronisation per phase:
+    //   1. inputs and extents            -> sync (extents)
+    //   2. grids (host), gridding        (no sync)
+    //   3. cell-pair flags + scans, counts gathered on the device -> sync
+    //   4. flags + scans again, records emitted into the concatenated list
     std::vector<DevLattice> hl(ncat);
-    std::vector<std::pair<CellPairRec*, int>> cpl(ncat);
-    double wrap0[3] = {0, 0, 0}, coord_scale = 0, cs_max = 0, tested_all = 0;
-    int rmaxr = 1, ncp_all = 0;
+    std::vector<const void*> dx(ncat), dy(ncat), dz(ncat), dw(ncat);
     for (int k = 0; k < ncat; k++) {
-        struct config_options oc = *options;
         const std::string tag = "b" + std::to_string(k);
         if (need_w && W[k] == nullptr) {
             fprintf(stderr, "Error: catalog %d has no weights\n", k);
             return EXIT_FAILURE;
         }
-        const void *x = to_device(tag + "x", X[k], ND[k], esize), *y = to_device(tag + "y", Y[k], ND[k], esize),
-                   *z = to_device(tag + "z", Z[k], ND[k], esize);
-        const void* w = need_w ? to_device(tag + "w", W[k], ND[k], esize) : nullptr;
-        double ext[6], wrap[3];
+        dx[k] = to_device(tag + "x", X[k], ND[k], esize);
+        dy[k] = to_device(tag + "y", Y[k], ND[k], esize);
+        dz[k] = to_device(tag + "z", Z[k], ND[k], esize);
+        dw[k] = need_w ? to_device(tag + "w", W[k], ND[k], esize) : nullptr;
+    }
+    if (options->periodic && options->boxsize == BOXSIZE_NOTGIVEN) {
+        fprintf(stderr, "boxsize must be specified with periodic wrap.\n");
+        return EXIT_FAILURE;
+    }
+    std::vector<double> exts(6 * (size_t)ncat);
+    {
+        double* d_ext = (double*)E.arena.get("b_ext", sizeof(double) * 6 * ncat);
+        for (int k = 0; k < ncat; k++) device_minmax_async(dx[k], dy[k], dz[k], ND[k], esize, d_ext + 6 * k);
+        CF_CHECK(hipMemcpyAsync(exts.data(), d_ext, sizeof(double) * 6 * ncat, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+    }
+    std::vector<std::array<int, 3>> refines(ncat), pers(ncat);
+    double wrap0[3] = {0, 0, 0}, coord_scale = 0, cs_max = 0, tested_all = 0;
+    int rmaxr = 1;
+    unsigned long long* d_oob = (unsigned long long*)E.arena.get("b_oob", sizeof(unsigned long long));
+    CF_CHECK(hipMemsetAsync(d_oob, 0, sizeof(unsigned long long), E.stream));
+    for (int k = 0; k < ncat; k++) {
+        struct config_options oc = *options;
+        const double* ext = &exts[6 * (size_t)k];
+        double wrap[3];
         int per[3];
-        device_minmax(x, y, z, ND[k], esize, ext);
-        if (oc.periodic && oc.boxsize == BOXSIZE_NOTGIVEN) {
-            fprintf(stderr, "boxsize must be specified with periodic wrap.\n");
-            return EXIT_FAILURE;
-        }
         const double by = oc.boxsize_y == BOXSIZE_NOTGIVEN ? oc.boxsize : oc.boxsize_y;
         const double bz = oc.boxsize_z == BOXSIZE_NOTGIVEN ? oc.boxsize : oc.boxsize_z;
         per[0] = oc.periodic && oc.boxsize_x >= 0;
@@ -1205,7 +1304,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         }
         if (E.autotune && get_bin_refine_scheme(&oc) == BINNING_DFL && ND[k] > 0) {
             // per size class: tuned on a single-catalog call of this catalog
-            std::array<int, 4> ch;
+            std::array<int, 5> ch;
             const std::string key = tune_key(MODE_DD, ND[k], 0, 1, rmax, rmax, ext, wrap, need_w, oc.need_avg_sep,
                                              esize, nbin);
             CallSpec cs{MODE_DD, ND[k], 0, X[k], Y[k], Z[k], nullptr, nullptr, nullptr, 1, binfile, 0, 0, 0, 0, 1};
@@ -1213,49 +1312,82 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
             ek.weights0.weights[0] = need_w ? W[k] : nullptr;
             if (autotune_choice(cs, &oc, &ek, key, ch) != EXIT_SUCCESS) return EXIT_FAILURE;
             set_custom_bin_refine_factors(&oc, ch.data());
+            if (ND[k] > batch_fetch_n) {
+                batch_fetch_n = ND[k];
+                batch_fetch = ch[4];
+            }
         }
         int refine[3];
         GridSpec g;
         if (choose_grid(MODE_DD, ND[k], rmax, rmax, 0.0, ext, wrap, &oc, g, refine) != EXIT_SUCCESS)
             return EXIT_FAILURE;
-        if (gridlink_device(tag, g, ND[k], x, y, z, w, esize, hl[k]) != EXIT_SUCCESS) return EXIT_FAILURE;
-        CellPairRec* cps = nullptr;
-        int ncp = 0;
-        double tested = 0;
-        if (cellpairs_device(hl[k], hl[k], refine, per, wrap, rmax, -1, -1, oc.enable_min_sep_opt, 1, &cps, &ncp,
-                             &tested, k) != EXIT_SUCCESS)
+        if (gridlink_launch("b" + std::to_string(k), g, ND[k], dx[k], dy[k], dz[k], dw[k], esize, hl[k], d_oob) !=
+            EXIT_SUCCESS)
             return EXIT_FAILURE;
-        CellPairRec* keep = (CellPairRec*)E.arena.get(tag + "cp", sizeof(CellPairRec) * (ncp + 1));
-        if (ncp > 0)
-            CF_CHECK(hipMemcpyAsync(keep, cps, sizeof(CellPairRec) * ncp, hipMemcpyDeviceToDevice, E.stream));
-        cpl[k] = {keep, ncp};
-        ncp_all += ncp;
-        tested_all += tested;
+        refines[k] = {refine[0], refine[1], refine[2]};
+        pers[k] = {per[0], per[1], per[2]};
         for (int d = 0; d < 6; d++) coord_scale = fmax(coord_scale, fabs(ext[d]));
         cs_max = fmax(cs_max, fmax(g.xbs, fmax(g.ybs, g.zbs)));
         for (int d = 0; d < 3; d++) rmaxr = refine[d] > rmaxr ? refine[d] : rmaxr;
     }
+    // cell-pair counts per catalog
+    std::vector<int> ncps(ncat);
+    {
+        int* d_ncp = (int*)E.arena.get("b_ncp", sizeof(int) * ncat);
+        for (int k = 0; k < ncat; k++) {
+            CpPlan cp;
+            if (cellpairs_flag(hl[k], hl[k], refines[k].data(), pers[k].data(), wrap0, rmax, -1, -1,
+                               options->enable_min_sep_opt, 1, cp) != EXIT_SUCCESS)
+                return EXIT_FAILURE;
+            k_copy_int<<<1, 64, 0, E.stream>>>(cp.pos + cp.total, d_ncp + k);
+        }
+        CF_CHECK(hipGetLastError());
+        unsigned long long h_oob = 0;
+        CF_CHECK(hipMemcpyAsync(ncps.data(), d_ncp, sizeof(int) * ncat, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipMemcpyAsync(&h_oob, d_oob, sizeof(h_oob), hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        if (check_oob(h_oob) != EXIT_SUCCESS) return EXIT_FAILURE;
+    }
+    int ncp_all = 0;
+    std::vector<int> cp_off(ncat);
+    for (int k = 0; k < ncat; k++) {
+        cp_off[k] = ncp_all;
+        if ((int64_t)ncp_all + ncps[k] >= (int64_t)INT32_MAX) {
+            fprintf(stderr, "Error: the batch's cell pairs exceed the engine's 32-bit indexing\n");
+            return EXIT_FAILURE;
+        }
+        ncp_all += ncps[k];
+    }
     coord_scale = 6 * coord_scale + wrap0[0] + wrap0[1] + wrap0[2] + rmax;
+    // concatenated work list, emitted in place
+    CellPairRec* cps = (CellPairRec*)E.arena.get("bcp_all", sizeof(CellPairRec) * ((size_t)ncp_all + 1));
+    double* d_tested = (double*)E.arena.get("b_tested", sizeof(double) * ncat);
+    CF_CHECK(hipMemsetAsync(d_tested, 0, sizeof(double) * ncat, E.stream));
+    for (int k = 0; k < ncat; k++) {
+        if (ncps[k] == 0) continue;
+        CpPlan cp;
+        if (cellpairs_flag(hl[k], hl[k], refines[k].data(), pers[k].data(), wrap0, rmax, -1, -1,
+                           options->enable_min_sep_opt, 1, cp) != EXIT_SUCCESS)
+            return EXIT_FAILURE;
+        cellpairs_emit(cp, cps + cp_off[k], d_tested + k, k);
+    }
+    {
+        std::vector<double> tk(ncat);
+        CF_CHECK(hipMemcpyAsync(tk.data(), d_tested, sizeof(double) * ncat, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
+        for (int k = 0; k < ncat; k++) tested_all += tk[k];
+    }
     const double t_grid = now_s();
 
-    // concatenated work list
-    CellPairRec* cps = (CellPairRec*)E.arena.get("bcp_all", sizeof(CellPairRec) * (ncp_all + 1));
-    {
-        int o = 0;
-        for (int k = 0; k < ncat; k++) {
-            if (cpl[k].second > 0)
-                CF_CHECK(hipMemcpyAsync(cps + o, cpl[k].first, sizeof(CellPairRec) * cpl[k].second,
-                                        hipMemcpyDeviceToDevice, E.stream));
-            o += cpl[k].second;
-        }
-    }
     DevLattice* lats = (DevLattice*)E.arena.get("blats", sizeof(DevLattice) * ncat);
     CF_CHECK(hipMemcpyAsync(lats, hl.data(), sizeof(DevLattice) * ncat, hipMemcpyHostToDevice, E.stream));
 
     PairParams P;
     std::vector<unsigned short> lut;
+    E.force_fetch = batch_fetch;
     build_params(MODE_DD, resolve_isa(options->instruction_set), nbin, 0, nslots, rupp_sqr, sorted, rmax, rmax, 0,
                  0, options->need_avg_sep, need_w, 1, wrap0, coord_scale, cs_max, rmaxr, P, lut);
+    E.force_fetch = 0;
 
     const size_t tot = (size_t)ncat * nslots;
     unsigned long long* g_np = (unsigned long long*)E.arena.get("bh_np", sizeof(uint64_t) * tot);

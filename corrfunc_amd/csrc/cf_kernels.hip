@@ -179,6 +179,7 @@ __global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restric
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int b = bucket[i];
+    if (b < 0) return;  // outside the grid: counted by k_bucket_count, reported by the host
     const int pos = bstart[b] + atomicAdd(&fill[b], 1);
     const double xv = (double)X[i], yv = (double)Y[i], zv = (double)Z[i];
     x[pos] = xv;
@@ -193,13 +194,18 @@ __global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restric
     f4[pos] = make_float4((float)(xv - ox), (float)(yv - oy), (float)(zv - oz), 0.f);
 }
 
+// Per-cell bounding boxes, one wave per cell (lanes stride over the cell's
+// particles, then a shuffle reduction; min/max are exact, so the order of the
+// reduction does not matter). Empty cells get (+DBL_MAX, -DBL_MAX).
 __global__ void k_cell_bounds(int64_t ncells, const int* __restrict__ bstart, const double* __restrict__ x,
                               const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ bounds)
 {
-    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
+    const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (c >= ncells) return;  // uniform per wave
     double b[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
-    for (int k = bstart[c * kSlabs]; k < bstart[(c + 1) * kSlabs]; k++) {
+    const int e = bstart[(c + 1) * kSlabs];
+    for (int k = bstart[c * kSlabs] + lane; k < e; k += 64) {
         b[0] = x[k] < b[0] ? x[k] : b[0];
         b[1] = x[k] > b[1] ? x[k] : b[1];
         b[2] = y[k] < b[2] ? y[k] : b[2];
@@ -207,7 +213,16 @@ __global__ void k_cell_bounds(int64_t ncells, const int* __restrict__ bstart, co
         b[4] = z[k] < b[4] ? z[k] : b[4];
         b[5] = z[k] > b[5] ? z[k] : b[5];
     }
-    for (int d = 0; d < 6; d++) bounds[c * 6 + d] = b[d];
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 6; d += 2) {
+            const double lo = __shfl_xor(b[d], o, 64), hi = __shfl_xor(b[d + 1], o, 64);
+            b[d] = lo < b[d] ? lo : b[d];
+            b[d + 1] = hi > b[d + 1] ? hi : b[d + 1];
+        }
+    }
+    if (lane == 0)
+        for (int d = 0; d < 6; d++) bounds[c * 6 + d] = b[d];
 }
 
 __global__ void k_weight_sums(int64_t n, const double* __restrict__ w, double* __restrict__ out)
@@ -1145,6 +1160,12 @@ __global__ void k_emit_chunks(int ncp, const int* __restrict__ order, const Cell
             items[o++] = w;
         }
     }
+}
+
+// dst[0] = src[0] (a device-side gather of a count computed by a scan).
+__global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
 }
 
 // *single = item index where the light cell pairs start: the work-list offset
