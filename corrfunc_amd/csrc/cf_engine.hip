@@ -712,7 +712,8 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     if (E.force_R > 0 && E.force_R < R) R = E.force_R;
     P.hist_copies = R;
     const size_t lds = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
-                       (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 5;
+                       (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 4 + 16 +
+                       (size_t)nwaves * 64 * sizeof(double4);
     if (lds > 64 * 1024) {
         fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
         return EXIT_FAILURE;
@@ -1147,7 +1148,9 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
 
     int nitems = 0;
     TunedRun T;
-    const bool tuned = ncp > 0 && E.engine_kind != CF_AMD_ENGINE_PLAIN;
+    // the tuned kernel packs secondary indices into 26 bits; larger secondary
+    // sets are counted by the plain kernel
+    const bool tuned = ncp > 0 && E.engine_kind != CF_AMD_ENGINE_PLAIN && L2.n < kMaxSecondary;
     if (tuned) {
         if (launch_tuned(ncp, cps, lats_for(L1, L2), 0, P, (int)nslots, nbin, d_edges, d_lut, g_np, g_avg, g_w,
                          c.cell_timer || E.model.recording, T) != EXIT_SUCCESS)
@@ -1260,6 +1263,11 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         const std::string tag = "b" + std::to_string(k);
         if (need_w && W[k] == nullptr) {
             fprintf(stderr, "Error: catalog %d has no weights\n", k);
+            return EXIT_FAILURE;
+        }
+        if (ND[k] >= kMaxSecondary) {
+            fprintf(stderr, "Error: catalog %d has %lld particles; the batched kernel takes fewer than %lld\n", k,
+                    (long long)ND[k], (long long)kMaxSecondary);
             return EXIT_FAILURE;
         }
         dx[k] = to_device(tag + "x", X[k], ND[k], esize);

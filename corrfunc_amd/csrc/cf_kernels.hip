@@ -589,12 +589,14 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 //    rejects pairs that cannot be in range; the survivors are compacted into a
 //    per-wave LDS queue and decided 64 at a time in FP64.
 // Either way every counted pair is decided by the reference's FP64 arithmetic.
-constexpr int kQueue = 128;
+constexpr int kQueue = 256;         // survivor ring per wave (entries (j << 6) | lane)
 constexpr int kQueues = 8;          // work sub-queues (one counter each)
 constexpr int kCounterStride = 32;  // ints between sub-queue counters (128 B)
 // Per-wave queue storage: kQueue ring entries plus one scratch entry per lane,
 // which lanes without a survivor write to (the enqueue is branch-free).
 constexpr int kQueueAlloc = kQueue + 64;
+// Secondary indices must fit the packed queue entry ((j << 6) | lane, 32 bits).
+constexpr int64_t kMaxSecondary = (int64_t)1 << 26;
 
 // Broadcast of lane k's value (k wave-uniform; reads the VGPR whatever the
 // execution mask).
@@ -729,26 +731,24 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
 
 template <int MODE, int R2F>
 __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const DevLattice& L1,
-                                            const DevLattice& L2, int i0, int istride, const int* qj,
-                                            const unsigned char* ql, int head, int ndrain, KernelLds& S)
+                                            const DevLattice& L2, int i0, int istride, const unsigned* qe,
+                                            const double4* prim, int head, int ndrain, KernelLds& S)
 {
     const int lane = lane_id();
     if (lane < ndrain) {
-        const int slotq = (head + lane) & (kQueue - 1);
-        const int j = qj[slotq];
-        const int i = i0 + ql[slotq] * istride;
-        const double4 pi = L1.d4[i];
+        const unsigned e = qe[(head + lane) & (kQueue - 1)];
+        const int j = (int)(e >> 6), l = (int)(e & 63);
+        const double4 pi = prim[l];  // the primary's wrapped position and weight
         const double4 pj = L2.d4[j];
-        exact_pair<MODE, R2F>(P, cp, L1, i, 1, pi.x + cp.wx * P.xwrap, pi.y + cp.wy * P.ywrap,
-                              pi.z + cp.wz * P.zwrap, pi.w, pj.x, pj.y, pj.z, pj.w, S);
+        exact_pair<MODE, R2F>(P, cp, L1, i0 + l * istride, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
 }
 
 // One work item of the tuned kernel (see the comment above kQueue).
 template <int MODE, int R2F>
 __device__ __forceinline__ void process_item(const PairParams& P, const DevLattice& L1, const DevLattice& L2,
-                                             const CellPairRec& cp, const WorkItem& it, KernelLds& S, int* qj,
-                                             unsigned char* ql)
+                                             const CellPairRec& cp, const WorkItem& it, KernelLds& S, unsigned* qe,
+                                             double4* prim)
 {
     const int lane = lane_id();
     const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
@@ -830,39 +830,50 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     const int istride = stride > 0 ? stride : 1;
     int head = 0, tail = 0, count = 0;
     const int same = cp.same;
+    // the wave's primaries (wrapped, FP64) for the drains
+    prim[lane] = make_double4(xpos, ypos, zpos, wi);
+    __builtin_amdgcn_wave_barrier();
     const float4* __restrict__ f4 = L2.f4;
     float4 cur = f4[min(jlo + lane, jlast)];
     for (int jb = jlo; jb < jhi; jb += 64) {
         const float4 nxt = f4[min(jb + 64 + lane, jlast)];
         const int mt = min(64, jhi - jb);
-        for (int k = 0; k < mt; k++) {
-            const int j = jb + k;
-            const float dx = bcast_f(cur.x, k) - xr, dy = bcast_f(cur.y, k) - yr, dz = bcast_f(cur.z, k) - zr_i;
-            float q;
-            int pass;
+        // two secondaries per step: j0 = jb + k, j1 = jb + k + 1 (valid when k + 1 < mt)
+        for (int k = 0; k < mt; k += 2) {
+            const int k1 = min(k + 1, 63);
+            const int j0 = jb + k, j1 = jb + k + 1;
+            const float dx0 = bcast_f(cur.x, k) - xr, dy0 = bcast_f(cur.y, k) - yr, dz0 = bcast_f(cur.z, k) - zr_i;
+            const float dx1 = bcast_f(cur.x, k1) - xr, dy1 = bcast_f(cur.y, k1) - yr, dz1 = bcast_f(cur.z, k1) - zr_i;
+            int p0, p1;
             if (MODE == MODE_RPPI || MODE == MODE_WP) {
-                q = fmaf(dy, dy, dx * dx);
-                pass = ivalid && q < hi2 && q >= lo2 && fabsf(dz) < zr;
+                const float q0 = fmaf(dy0, dy0, dx0 * dx0), q1 = fmaf(dy1, dy1, dx1 * dx1);
+                p0 = q0 < hi2 && q0 >= lo2 && fabsf(dz0) < zr;
+                p1 = q1 < hi2 && q1 >= lo2 && fabsf(dz1) < zr;
             } else {
-                q = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                pass = ivalid && q < hi2 && q >= lo2;
+                const float q0 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+                const float q1 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+                p0 = q0 < hi2 && q0 >= lo2;
+                p1 = q1 < hi2 && q1 >= lo2;
             }
-            if (same) pass = pass && (j > i);
-            const unsigned long long m = __ballot(pass);
-            if (m) {
+            p0 = p0 && ivalid && (!same || j0 > i);
+            p1 = p1 && ivalid && (k + 1 < mt) && (!same || j1 > i);
+            const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+            if (m0 | m1) {
                 // branch-free enqueue: survivors take consecutive ring entries,
                 // the other lanes write their scratch entry
-                const int before =
-                    __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-                const int pos = pass ? ((tail + before) & (kQueue - 1)) : (kQueue + lane);
-                qj[pos] = j;
-                ql[pos] = (unsigned char)lane;
-                const int added = __popcll(m);
+                const int b0 = __builtin_amdgcn_mbcnt_hi((unsigned)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m0, 0));
+                const int b1 = __builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m1, 0));
+                const int n0 = __popcll(m0);
+                const int pos0 = p0 ? ((tail + b0) & (kQueue - 1)) : (kQueue + lane);
+                const int pos1 = p1 ? ((tail + n0 + b1) & (kQueue - 1)) : (kQueue + lane);
+                qe[pos0] = ((unsigned)j0 << 6) | (unsigned)lane;
+                qe[pos1] = ((unsigned)j1 << 6) | (unsigned)lane;
+                const int added = n0 + __popcll(m1);
                 tail += added;
                 count += added;
-                if (count >= 64) {
+                while (count >= 64) {
                     __builtin_amdgcn_wave_barrier();
-                    drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qj, ql, head, 64, S);
+                    drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, 64, S);
                     __builtin_amdgcn_wave_barrier();
                     head += 64;
                     count -= 64;
@@ -873,7 +884,7 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     }
     if (count > 0) {
         __builtin_amdgcn_wave_barrier();
-        drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qj, ql, head, count, S);
+        drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, count, S);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -938,8 +949,10 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
     S.e = e;
     S.lut = lut;
-    int* qj = qbase + wave * kQueueAlloc;
-    unsigned char* ql = reinterpret_cast<unsigned char*>(qbase + nwaves * kQueueAlloc) + wave * kQueueAlloc;
+    unsigned* qe = reinterpret_cast<unsigned*>(qbase) + wave * kQueueAlloc;
+    // primaries cache: 16-byte aligned after the queues
+    const uintptr_t pa = (reinterpret_cast<uintptr_t>(qbase + nwaves * kQueueAlloc) + 15) & ~(uintptr_t)15;
+    double4* prim = reinterpret_cast<double4*>(pa) + wave * 64;
     for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
     for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
     for (int s = threadIdx.x; s < nreg; s += blockDim.x) {
@@ -1009,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
         // (branch-free in the lanes: lane 0 adds dt, the others 0; every lane
         // stores the same compute-unit id)
         const unsigned long long t0 = cp_ticks ? wall_clock64() : 0ull;
-        process_item<MODE, R2F>(P, L1, L2, cp, it, S, qj, ql);
+        process_item<MODE, R2F>(P, L1, L2, cp, it, S, qe, prim);
         if (cp_ticks) {
             const unsigned long long dt = wall_clock64() - t0;
             atomicAdd(&cp_ticks[it.cp], lane == 0 ? dt : 0ull);
