@@ -76,7 +76,7 @@ struct PairParams {
     int dense_stride;        // 1: dense work items take strided primaries
     float dense_frac;        // a cell pair whose estimated in-range fraction reaches this takes the dense path
     int fetch;               // work items per queue atomic outside the single-fetch ranges
-    int pad_i3;
+    int prim_cache;          // 1: the sparse path keeps the wave's primaries in LDS for its drains
 };
 
 // Work item of the tuned kernel: (cell pair, <= 64 primary particles, one part

@@ -711,15 +711,21 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     }
     if (E.force_R > 0 && E.force_R < R) R = E.force_R;
     P.hist_copies = R;
-    const size_t lds = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
-                       (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 4 + 16 +
-                       (size_t)nwaves * 64 * sizeof(double4);
+    const size_t lds_base = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
+                            (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 4;
+    const size_t lds_prim = 16 + (size_t)nwaves * 64 * sizeof(double4);
+    auto blocks_per_cu = [](size_t l) {
+        const int b = (int)((160 * 1024) / (l + 1024));
+        return b < 1 ? 1 : (b > 8 ? 8 : b);
+    };
+    // the primaries cache only where it costs no workgroup per CU
+    P.prim_cache = blocks_per_cu(lds_base + lds_prim) >= blocks_per_cu(lds_base) ? 1 : 0;
+    const size_t lds = lds_base + (P.prim_cache ? lds_prim : 0);
     if (lds > 64 * 1024) {
         fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
         return EXIT_FAILURE;
     }
-    int per_cu = (int)((160 * 1024) / (lds + 1024));
-    per_cu = per_cu < 1 ? 1 : (per_cu > 8 ? 8 : per_cu);
+    const int per_cu = blocks_per_cu(lds);
     const long max_blocks = (long)E.ncu * per_cu;
 
     // ---- cost prediction and order

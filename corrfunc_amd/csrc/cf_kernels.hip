@@ -738,9 +738,18 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
     if (lane < ndrain) {
         const unsigned e = qe[(head + lane) & (kQueue - 1)];
         const int j = (int)(e >> 6), l = (int)(e & 63);
-        const double4 pi = prim[l];  // the primary's wrapped position and weight
+        const int i = i0 + l * istride;
+        double4 pi;
+        if (P.prim_cache) {
+            pi = prim[l];  // the primary's wrapped position and weight
+        } else {
+            pi = L1.d4[i];
+            pi.x += cp.wx * P.xwrap;
+            pi.y += cp.wy * P.ywrap;
+            pi.z += cp.wz * P.zwrap;
+        }
         const double4 pj = L2.d4[j];
-        exact_pair<MODE, R2F>(P, cp, L1, i0 + l * istride, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
+        exact_pair<MODE, R2F>(P, cp, L1, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
 }
 
@@ -831,8 +840,10 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     int head = 0, tail = 0, count = 0;
     const int same = cp.same;
     // the wave's primaries (wrapped, FP64) for the drains
-    prim[lane] = make_double4(xpos, ypos, zpos, wi);
-    __builtin_amdgcn_wave_barrier();
+    if (P.prim_cache) {
+        prim[lane] = make_double4(xpos, ypos, zpos, wi);
+        __builtin_amdgcn_wave_barrier();
+    }
     const float4* __restrict__ f4 = L2.f4;
     float4 cur = f4[min(jlo + lane, jlast)];
     for (int jb = jlo; jb < jhi; jb += 64) {
