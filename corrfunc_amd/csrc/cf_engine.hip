@@ -711,21 +711,41 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     }
     if (E.force_R > 0 && E.force_R < R) R = E.force_R;
     P.hist_copies = R;
+    const void* kfn = nullptr;
+#define CF_K(M, F, B) if (P.mode == M && P.r2form == F && batch == B) kfn = (const void*)&k_pairs_tuned<M, F, B>;
+    CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
+    CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
+    CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
+    CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
+    CF_K(MODE_DD, R2_SUM_XY_Z, 1) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 1)
+#undef CF_K
+    if (!kfn) {
+        fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d%s\n", P.mode, P.r2form,
+                batch ? " (batched)" : "");
+        return EXIT_FAILURE;
+    }
     const size_t lds_base = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
                             (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 4;
     const size_t lds_prim = 16 + (size_t)nwaves * 64 * sizeof(double4);
-    auto blocks_per_cu = [](size_t l) {
-        const int b = (int)((160 * 1024) / (l + 1024));
+    // resident workgroups per CU of this kernel (registers and LDS), which is
+    // also how many persistent workgroups per CU are launched
+    auto resident = [&](size_t l) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kfn, threads, l) != hipSuccess) {
+            (void)hipGetLastError();
+            b = (int)((160 * 1024) / (l + 1024));
+        }
         return b < 1 ? 1 : (b > 8 ? 8 : b);
     };
-    // the primaries cache only where it costs no workgroup per CU
-    P.prim_cache = blocks_per_cu(lds_base + lds_prim) >= blocks_per_cu(lds_base) ? 1 : 0;
+    const int res_base = resident(lds_base);
+    // the primaries cache only where it costs no resident workgroup
+    P.prim_cache = resident(lds_base + lds_prim) >= res_base ? 1 : 0;
     const size_t lds = lds_base + (P.prim_cache ? lds_prim : 0);
     if (lds > 64 * 1024) {
         fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
         return EXIT_FAILURE;
     }
-    const int per_cu = blocks_per_cu(lds);
+    const int per_cu = resident(lds);
     const long max_blocks = (long)E.ncu * per_cu;
 
     // ---- cost prediction and order
@@ -808,19 +828,6 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         k_fill_i32<<<nb, 256, 0, E.stream>>>(ncp, cu, -1);
     }
 
-    const void* kfn = nullptr;
-#define CF_K(M, F, B) if (P.mode == M && P.r2form == F && batch == B) kfn = (const void*)&k_pairs_tuned<M, F, B>;
-    CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
-    CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
-    CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
-    CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
-    CF_K(MODE_DD, R2_SUM_XY_Z, 1) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 1)
-#undef CF_K
-    if (!kfn) {
-        fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d%s\n", P.mode, P.r2form,
-                batch ? " (batched)" : "");
-        return EXIT_FAILURE;
-    }
     void* kargs[] = {&P, &lats, &cps, &items, &d_nitems, &counter, &d_edges, &d_lut, &g_np, &g_avg, &g_w, &ticks, &cu, &single};
     CF_CHECK(hipEventRecord(E.ev0, E.stream));
     CF_CHECK(hipLaunchKernel(kfn, dim3((unsigned)blocks), dim3(threads), kargs, lds, E.stream));
