@@ -701,16 +701,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     Engine& E = eng();
     const int threads = 256, nwaves = threads / 64;
     const int regions = batch ? nwaves : 1;
-    // histogram replication: as many copies (power of 2, <= 32) as fit a 40 KiB budget
     const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
-    int R = 1;
-    while (R < 32 && (size_t)regions * nslots * per_slot * (2 * R) <= 40 * 1024) R *= 2;
-    if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
-        const int v = atoi(rc);
-        if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
-    }
-    if (E.force_R > 0 && E.force_R < R) R = E.force_R;
-    P.hist_copies = R;
     const void* kfn = nullptr;
 #define CF_K(M, F, B) if (P.mode == M && P.r2form == F && batch == B) kfn = (const void*)&k_pairs_tuned<M, F, B>;
     CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
@@ -724,9 +715,9 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
                 batch ? " (batched)" : "");
         return EXIT_FAILURE;
     }
-    const size_t lds_base = (size_t)regions * nslots * per_slot * R + (size_t)nbin * 8 +
-                            (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 4;
+    const size_t lds_fixed = (size_t)nbin * 8 + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 4;
     const size_t lds_prim = 16 + (size_t)nwaves * 64 * sizeof(double4);
+    auto lds_for = [&](int r) { return (size_t)regions * nslots * per_slot * r + lds_fixed; };
     // resident workgroups per CU of this kernel (registers and LDS), which is
     // also how many persistent workgroups per CU are launched
     auto resident = [&](size_t l) {
@@ -737,6 +728,22 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         }
         return b < 1 ? 1 : (b > 8 ? 8 : b);
     };
+    // LDS histogram replication R (power of 2, <= 32, histograms <= 40 KiB): the
+    // largest that keeps min(4, best achievable) workgroups per CU resident
+    const int target = std::min(4, resident(lds_for(1)));
+    int R = 1;
+    for (int r = 2; r <= 32; r *= 2) {
+        if ((size_t)regions * nslots * per_slot * r > 40 * 1024) break;
+        if (resident(lds_for(r)) < target) break;
+        R = r;
+    }
+    if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
+        const int v = atoi(rc);
+        if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
+    }
+    if (E.force_R > 0 && E.force_R < R) R = E.force_R;
+    P.hist_copies = R;
+    const size_t lds_base = lds_for(R);
     const int res_base = resident(lds_base);
     // the primaries cache only where it costs no resident workgroup
     P.prim_cache = resident(lds_base + lds_prim) >= res_base ? 1 : 0;
