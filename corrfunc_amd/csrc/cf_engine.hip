@@ -747,6 +747,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     const int res_base = resident(lds_base);
     // the primaries cache only where it costs no resident workgroup
     P.prim_cache = resident(lds_base + lds_prim) >= res_base ? 1 : 0;
+    if (const char* pc = getenv("CORRFUNC_AMD_PRIM_CACHE")) P.prim_cache = atoi(pc) != 0;
     const size_t lds = lds_base + (P.prim_cache ? lds_prim : 0);
     if (lds > 64 * 1024) {
         fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);

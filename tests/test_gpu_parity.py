@@ -134,3 +134,45 @@ def test_device_copy(engine, nbytes):
     assert np.array_equal(b.numpy(), src)
     a.free()
     b.free()
+
+
+# Engine variables of the tuned kernel's work list and paths (read on every call).
+KNOBS = [
+    {"CORRFUNC_AMD_FETCH": "4"},
+    {"CORRFUNC_AMD_FETCH": "3", "CORRFUNC_AMD_HEAVY": "1e9"},
+    {"CORRFUNC_AMD_DENSE_FRAC": "0"},
+    {"CORRFUNC_AMD_DENSE_FRAC": "2", "CORRFUNC_AMD_PRIM_CACHE": "0"},
+    {"CORRFUNC_AMD_PRIM_CACHE": "1", "CORRFUNC_AMD_HIST_COPIES": "1"},
+    {"CORRFUNC_AMD_SPLIT": "64", "CORRFUNC_AMD_HEAVY": "1"},
+    {"CORRFUNC_AMD_SPLIT": "0", "CORRFUNC_AMD_DENSE": "2"},
+]
+
+
+@pytest.mark.parametrize("knobs", KNOBS, ids=lambda k: ",".join("%s=%s" % (a[13:], b) for a, b in k.items()))
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("sched", ["none", "heuristic"])
+def test_engine_variables_keep_counts(engine, oracle, knobs, mode, sched):
+    """Fetch batches, sub-queue stealing, dense/sparse choice, the primaries cache,
+    histogram replication, heavy-first order and window splitting never change a count."""
+    from corrfunc_amd import scheduler
+    _select("tuned")
+    L = 160.0
+    X = clustered(40000, L, 91, nparent=25, sigma=2.5, dtype=np.float64)
+    W = weights(40000, 92, np.float64)
+    bins = np.concatenate([[0.0], np.logspace(np.log10(0.2), np.log10(15.0), 11)])
+    kw = mode_kwargs(mode, X, W, autocorr=1, bins=bins, L=L, pimax=12.0, mu_max=1.0, nmu_bins=9)
+    o = dict(periodic=True, boxsize=L, need_avg_sep=True, isa="avx512f")
+    scheduler.set_scheduler(sched)
+    old = {k: os.environ.get(k) for k in knobs}
+    try:
+        os.environ.update(knobs)
+        got = run(engine, mode, options=_opts(**o), **kw)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+        scheduler.set_scheduler("learned")
+    want = run(oracle, mode, "oracle_", options=_opts(**o), **kw)
+    assert_same(mode, got, want, what=str(knobs))
