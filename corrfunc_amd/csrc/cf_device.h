@@ -20,6 +20,11 @@ enum R2Form {
 // Number of z-slabs each cell is bucketed into by the gridding sort.
 constexpr int kSlabs = 16;
 
+// Quantized positions: |q| <= kQMax, so differences fit int16 and a sum of three
+// squared differences fits uint32; kQPad entries pad the array for block loads.
+constexpr int kQMax = 16000;
+constexpr int kQPad = 8;
+
 // One neighbour-cell pair of the work list (64 bytes). Mirrors the fields of
 // the reference's struct cell_pair (utils/cell_pair.h.src:20-37); the wrap
 // offsets are stored as signs so off = sign * wrap is exact.
@@ -42,7 +47,15 @@ struct DevLattice {
     double xinv, yinv, zinv;   // 1/cell size (0 if the size is 0)
     double *x, *y, *z, *w;     // FP64 positions / weight, cell order
     double4 *d4;               // the same as one 32-byte record (x, y, z, w) per particle
-    float4 *f4;                // cell-local FP32 (x - ox, y - oy, z - oz, 0), ox = xmin + ix*xbs
+    // Cell-local quantized positions for the integer pair filter: per particle
+    // (qx & 0xffff) | (qy << 16) and qz, q = rint((x - ox) * qinv), ox = xmin + ix*xbs,
+    // followed by kQPad padding entries.
+    uint2 *q;
+    double qinv;               // quanta per unit length: kQMax quanta span a cell plus the pair reach
+    unsigned qrange;           // filter: pass iff (dq^2 + qlo_neg) mod 2^32 < qrange
+    int qlo_neg;
+    int qz;                    // projected modes: pass iff |dqz| < qz
+    int qpad_;
     int *bstart;               // [ncells*kSlabs + 1] bucket starts
     double *bounds;            // [ncells*6] x lo,hi, y lo,hi, z lo,hi
 };
@@ -64,8 +77,6 @@ struct PairParams {
     double max_all_dz_same;  // same-cell z window (DD/xi/s_mu)
     double xwrap, ywrap, zwrap;
     double near_r2, near_dz; // edge windows below which pruning replay is skipped
-    // FP32 filter thresholds
-    float hi2_f, lo2_f, zreach_f, pad_f;
     double zreach;           // z reach used for the slab window
     double slab_margin;
     // bin lookup: slot count of edges below the 1/16-octave bucket of (float)r^2

@@ -255,6 +255,7 @@ struct GridSpec {
     int nx, ny, nz;
     double xmin, ymin, zmin, xmax, ymax, zmax;
     double xbs, ybs, zbs, xinv, yinv, zinv;
+    double qreach;  // largest separation a pair may have along one axis (quantization range)
 };
 
 // gridlink_utils.c.src:31-49
@@ -316,12 +317,15 @@ static int gridlink_launch(const std::string& tag, const GridSpec& g, int64_t n,
     L.y = (double*)E.arena.get(tag + "_y", sizeof(double) * n);
     L.z = (double*)E.arena.get(tag + "_z", sizeof(double) * n);
     L.w = W ? (double*)E.arena.get(tag + "_w", sizeof(double) * n) : nullptr;
-    L.d4 = (double4*)E.arena.get(tag + "_d4", sizeof(double4) * n);
-    L.f4 = (float4*)E.arena.get(tag + "_f4", sizeof(float4) * n);
+    L.d4 = (double4*)E.arena.get(tag + "_d4", sizeof(double4) * (n + kQPad));
+    // quantum: kQMax quanta cover the largest cell side plus the pair reach, so
+    // every position within reach of a cell has an unclamped quantized offset
+    L.qinv = (double)kQMax / (fmax(g.xbs, fmax(g.ybs, g.zbs)) + g.qreach);
+    L.q = (uint2*)E.arena.get(tag + "_q", sizeof(uint2) * (n + kQPad));
     L.bounds = (double*)E.arena.get(tag + "_bounds", sizeof(double) * 6 * L.ncells);
     CF_CHECK(hipMemsetAsync(counts, 0, sizeof(int) * (nb + 1), E.stream));
     CF_CHECK(hipMemsetAsync(fill, 0, sizeof(int) * (nb + 1), E.stream));
-    GridArgs ga{g.xmin, g.ymin, g.zmin, g.xinv, g.yinv, g.zinv, g.xbs, g.ybs, g.zbs, g.nx, g.ny, g.nz};
+    GridArgs ga{g.xmin, g.ymin, g.zmin, g.xinv, g.yinv, g.zinv, g.xbs, g.ybs, g.zbs, g.nx, g.ny, g.nz, L.qinv};
     const unsigned blocks = (unsigned)((n + 255) / 256);
     if (n > 0) {
         if (esize == 4)
@@ -337,11 +341,11 @@ static int gridlink_launch(const std::string& tag, const GridSpec& g, int64_t n,
         if (esize == 4)
             k_scatter<float><<<blocks, 256, 0, E.stream>>>(n, (const float*)X, (const float*)Y, (const float*)Z,
                                                             (const float*)W, ga, bucket, L.bstart, fill, L.x, L.y,
-                                                            L.z, L.w, L.d4, L.f4);
+                                                            L.z, L.w, L.d4, L.q);
         else
             k_scatter<double><<<blocks, 256, 0, E.stream>>>(n, (const double*)X, (const double*)Y, (const double*)Z,
                                                              (const double*)W, ga, bucket, L.bstart, fill, L.x,
-                                                             L.y, L.z, L.w, L.d4, L.f4);
+                                                             L.y, L.z, L.w, L.d4, L.q);
         CF_CHECK(hipGetLastError());
     }
     k_cell_bounds<<<(unsigned)((L.ncells + 3) / 4), 256, 0, E.stream>>>(L.ncells, L.bstart, L.x, L.y, L.z,
@@ -502,8 +506,6 @@ static int r2form_for(int mode, int isa_id)
     return R2_SUM_XY_Z;
 }
 
-static inline float f32_up(double v) { return nextafterf((float)v, INFINITY); }
-static inline float f32_down(double v) { return nextafterf((float)v, -INFINITY); }
 
 // Refine heuristics, grid, boost (countpairs_impl.c.src:272-332 and the per-mode
 // variants: rp_pi_impl:277-329, s_mu_impl:276-340, xi_impl:212-270, wp_impl:240-290).
@@ -553,6 +555,7 @@ static int choose_grid(int mode, int64_t N1, double rmax, double pimax, double b
     refine[1] = o->bin_refine_factors[1];
     refine[2] = o->bin_refine_factors[2];
     if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+    g.qreach = fmax(rmax, pimax) * (1.0 + 1e-6);
 
     return EXIT_SUCCESS;
 }
@@ -594,19 +597,8 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
     // separation lies within a few ulps of the coordinate scale below rmax / pimax.
     P.near_r2 = 1e-9 * P.sqr_rmax + 1e-9 * rmax * coord_scale;
     P.near_dz = 1e-9 * pimax + 1e-9 * coord_scale;
-    // FP32 filter margin: |dx_f32 - dx| <= eps for cell-local coordinates.
-    {
-        const double B = (rmaxr + 3) * cs + rmax + pimax;
-        const double eps = 8.0 * B * 0x1p-24;
-        const double r_hi = sqrt(P.sqr_rmax), r_lo = sqrt(P.sqr_rmin);
-        const double m_hi = 2.0 * sqrt(3.0) * r_hi * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmax;
-        const double m_lo = 2.0 * sqrt(3.0) * r_lo * eps + 3.0 * eps * eps + 8.0 * 0x1p-24 * P.sqr_rmin;
-        P.hi2_f = f32_up(P.sqr_rmax + m_hi);
-        P.lo2_f = P.sqr_rmin - m_lo > 0 ? f32_down(P.sqr_rmin - m_lo) : -INFINITY;
-        P.zreach_f = f32_up(pimax + 4.0 * eps);
-        P.zreach = (is2d || mode == MODE_SMU) ? pimax : sqrt(P.sqr_rmax);
-        P.slab_margin = 1e-7 * (coord_scale + cs);
-    }
+    P.zreach = (is2d || mode == MODE_SMU) ? pimax : sqrt(P.sqr_rmax);
+    P.slab_margin = 1e-7 * (coord_scale + cs);
 
     // ---- bin lookup table: for each 1/16-octave bucket of (float)r^2, the number
     // of edges strictly below the bucket (a lower bound of the bin index)
@@ -656,6 +648,23 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
 }
 
 // Device copy of the (primary, secondary) lattice pair of a single call.
+// Thresholds of the integer pair filter of a secondary set (quantum 1/L.qinv).
+// Rounding both positions of a pair to the nearest quantum moves their
+// difference by at most one quantum per axis, so in quanta a pair at distance d
+// has |dq| <= d*qinv + sqrt(3) (3-D) and |dqz| <= |dz|*qinv + 1; the bounds
+// below add slack to these, so every pair the FP64 test can accept passes.
+static void set_quant_filter(DevLattice& L, const PairParams& P)
+{
+    const bool is2d = P.mode == MODE_RPPI || P.mode == MODE_WP;
+    const double rhi = sqrt(P.sqr_rmax) * (1.0 + 1e-9) * L.qinv + 2.0;
+    const double rlo = sqrt(P.sqr_rmin) * (1.0 - 1e-9) * L.qinv - 2.0;
+    const double hi = floor(rhi * rhi) + 1.0;
+    const double lo = rlo > 0 ? floor(rlo * rlo) : 0.0;
+    L.qrange = (unsigned)(hi - lo);
+    L.qlo_neg = -(int)lo;
+    L.qz = is2d ? (int)ceil(P.pimax * (1.0 + 1e-9) * L.qinv + 1.0) + 1 : 0;
+}
+
 static const DevLattice* lats_for(const DevLattice& L1, const DevLattice& L2)
 {
     Engine& E = eng();
@@ -1148,6 +1157,8 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
                                                 : (refine[1] > refine[2] ? refine[1] : refine[2]);
         build_params(mode, resolve_isa(o->instruction_set), nbin, n2, (int)nslots, rupp_sqr, sorted, rmax, pimax,
                      inv_d2, sqr_mu_max, o->need_avg_sep, need_w, c.autocorr, wrap, coord_scale, cs, rmaxr, P, lut);
+        set_quant_filter(L1, P);
+        set_quant_filter(L2, P);
     }
 
     // ---- histograms
@@ -1408,15 +1419,15 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     }
     const double t_grid = now_s();
 
-    DevLattice* lats = (DevLattice*)E.arena.get("blats", sizeof(DevLattice) * ncat);
-    CF_CHECK(hipMemcpyAsync(lats, hl.data(), sizeof(DevLattice) * ncat, hipMemcpyHostToDevice, E.stream));
-
     PairParams P;
     std::vector<unsigned short> lut;
     E.force_fetch = batch_fetch;
     build_params(MODE_DD, resolve_isa(options->instruction_set), nbin, 0, nslots, rupp_sqr, sorted, rmax, rmax, 0,
                  0, options->need_avg_sep, need_w, 1, wrap0, coord_scale, cs_max, rmaxr, P, lut);
     E.force_fetch = 0;
+    for (int k = 0; k < ncat; k++) set_quant_filter(hl[k], P);
+    DevLattice* lats = (DevLattice*)E.arena.get("blats", sizeof(DevLattice) * ncat);
+    CF_CHECK(hipMemcpyAsync(lats, hl.data(), sizeof(DevLattice) * ncat, hipMemcpyHostToDevice, E.stream));
 
     const size_t tot = (size_t)ncat * nslots;
     unsigned long long* g_np = (unsigned long long*)E.arena.get("bh_np", sizeof(uint64_t) * tot);

@@ -80,7 +80,20 @@ struct GridArgs {
     double xinv, yinv, zinv;
     double xbs, ybs, zbs;
     int nx, ny, nz;
+    double qinv;  // quanta per unit length of the quantized positions
 };
+
+// Quantized cell-local coordinate: rint(v * qinv) clamped to [-kQMax, kQMax].
+__device__ __forceinline__ int quantize(double v, double qinv)
+{
+    const double t = fmin(fmax(v * qinv, -(double)kQMax), (double)kQMax);
+    return (int)rint(t);
+}
+__device__ __forceinline__ uint2 quantize_pos(double dx, double dy, double dz, double qinv)
+{
+    const int qx = quantize(dx, qinv), qy = quantize(dy, qinv), qz = quantize(dz, qinv);
+    return make_uint2(((unsigned)qx & 0xffffu) | ((unsigned)qy << 16), (unsigned)qz);
+}
 
 // Cell index exactly as the reference computes it (gridlink_impl.c.src:167-178),
 // followed by the z-slab inside the cell.
@@ -174,7 +187,7 @@ __global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restric
                           const T* __restrict__ W, GridArgs g, const int* __restrict__ bucket,
                           const int* __restrict__ bstart, int* __restrict__ fill,
                           double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
-                          double* __restrict__ w, double4* __restrict__ d4, float4* __restrict__ f4)
+                          double* __restrict__ w, double4* __restrict__ d4, uint2* __restrict__ q)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -191,7 +204,7 @@ __global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restric
     const int c = b / kSlabs;
     const int iz = c % g.nz, ixy = c / g.nz, iy = ixy % g.ny, ix = ixy / g.ny;
     const double ox = g.xmin + ix * g.xbs, oy = g.ymin + iy * g.ybs, oz = g.zmin + iz * g.zbs;
-    f4[pos] = make_float4((float)(xv - ox), (float)(yv - oy), (float)(zv - oz), 0.f);
+    q[pos] = quantize_pos(xv - ox, yv - oy, zv - oz, g.qinv);
 }
 
 // Per-cell bounding boxes, one wave per cell (lanes stride over the cell's
@@ -579,38 +592,42 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 // Work item: (cell pair, chunk of <= 64 consecutive primary particles).
 // Each wave pulls work items from a device counter until the list is exhausted.
 // Lane l holds primary particle i0+l. The secondary particles of the cell pair,
-// restricted to the z-slabs the chunk can reach, are streamed in tiles of 64:
-// one coalesced vector load per tile (lane l loads secondary jb+l; the next
-// tile is loaded while the current one is processed), then each secondary is
-// broadcast to all lanes with readlane. Two paths per work item:
+// restricted to the z-slabs the chunk can reach, are read with scalar loads: the
+// index is wave-uniform, so a secondary lands in scalar registers that every
+// lane's vector instructions read directly (no per-lane load, no broadcast
+// instruction). Two paths per work item:
 //  * dense (every pair of the cell pair is inside the separation range): each
 //    (i, j) is decided in FP64 in lockstep from a 32-byte (x, y, z, w) record;
-//  * sparse: an FP32 test on cell-local coordinates, with a proven error margin,
-//    rejects pairs that cannot be in range; the survivors are compacted into a
-//    per-wave LDS queue and decided 64 at a time in FP64.
+//  * sparse: an integer test on quantized cell-local positions (packed 16-bit
+//    differences, one dot-product instruction per pair), with thresholds that
+//    provably pass every pair the FP64 test can accept, rejects the pairs that
+//    cannot be in range; the survivors are compacted into a per-wave LDS queue
+//    and decided 64 at a time in FP64.
 // Either way every counted pair is decided by the reference's FP64 arithmetic.
-constexpr int kQueue = 256;         // survivor ring per wave (entries (j << 6) | lane)
+constexpr int kQueue = 512;         // survivor ring per wave (entries (j << 6) | lane)
 constexpr int kQueues = 8;          // work sub-queues (one counter each)
 constexpr int kCounterStride = 32;  // ints between sub-queue counters (128 B)
-// Per-wave queue storage: kQueue ring entries plus one scratch entry per lane,
-// which lanes without a survivor write to (the enqueue is branch-free).
-constexpr int kQueueAlloc = kQueue + 64;
+constexpr int kBlockJ = 4;          // secondaries per scalar load of the sparse path
+// Per-wave queue storage. Before a block of kBlockJ secondaries is enqueued the
+// ring holds < 64 entries, after it < 64 + 64 kBlockJ.
+constexpr int kQueueAlloc = kQueue;
+static_assert(64 + 64 * kBlockJ <= kQueue, "survivor ring too small for one block");
 // Secondary indices must fit the packed queue entry ((j << 6) | lane, 32 bits).
 constexpr int64_t kMaxSecondary = (int64_t)1 << 26;
 
-// Broadcast of lane k's value (k wave-uniform; reads the VGPR whatever the
-// execution mask).
-__device__ __forceinline__ float bcast_f(float v, int k)
+// Read-only particle data seen through the constant address space: loads with a
+// wave-uniform index become scalar loads.
+typedef const __attribute__((address_space(4))) double KDouble;
+typedef const __attribute__((address_space(4))) unsigned KUint;
+__device__ __forceinline__ double4 kload_d4(KDouble* p, int j)
 {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+    return make_double4(p[4 * (int64_t)j], p[4 * (int64_t)j + 1], p[4 * (int64_t)j + 2], p[4 * (int64_t)j + 3]);
 }
-__device__ __forceinline__ double bcast_d(double v, int k)
+__device__ __forceinline__ uint2 kload_u2(KUint* p, int j)
 {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, k);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), k);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    return make_uint2(p[2 * (int64_t)j], p[2 * (int64_t)j + 1]);
 }
+typedef short short2v __attribute__((ext_vector_type(2)));
 
 // LDS histogram replicated R ways with layout [slot][R]: lanes of one wave that
 // land in the same slot add to different copies, which sit in different banks.
@@ -637,12 +654,13 @@ __device__ __forceinline__ int lut_kbin(const PairParams& P, const KernelLds& S,
     return k;
 }
 
-// sqrt(r2) to ~1e-14 relative: FP32 square root refined by one Newton step in FP64.
+// sqrt(r2) to ~1e-14 relative: the hardware FP32 square root (about 1 ulp)
+// refined by one Newton step in FP64.
 __device__ __forceinline__ double fast_sqrt(double r2)
 {
     const float f = (float)r2;
     if (!(f > 1e-30f)) return sqrt(r2);
-    const float s = __builtin_sqrtf(f);
+    const float s = __builtin_amdgcn_sqrtf(f);
     const double sd = (double)s;
     const double res = fma(-sd, sd, r2);
     const double half_inv = 0.5 * (double)__builtin_amdgcn_rcpf(s);
@@ -732,7 +750,7 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
 template <int MODE, int R2F>
 __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const DevLattice& L1,
                                             const DevLattice& L2, int i0, int istride, const unsigned* qe,
-                                            const double4* prim, int head, int ndrain, KernelLds& S)
+                                            const double4* prim, unsigned head, int ndrain, KernelLds& S)
 {
     const int lane = lane_id();
     if (lane < ndrain) {
@@ -753,6 +771,96 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
     }
 }
 
+// Integer pair filter on quantized cell-local positions (secondary s, primary
+// qxy / qz). 3-D: (dqx^2 + dqy^2 + dqz^2 + qlo) mod 2^32 < qrange; projected:
+// the same on (dqx, dqy) and |dqz| < zq, as (dqz + zq - 1) unsigned < 2 zq - 1
+// with zoff = zq - 1 - qz. Differences fit 16 bits and sums of squares 32 bits
+// (|q| <= kQMax), so the only rounding is that of the positions themselves.
+struct QFilter {
+    unsigned qxy;
+    int qz, qlo, zoff;
+    unsigned qrange, zspan;
+    int ilim;                  // same cell: the lane takes j > ilim
+    bool valid;                // the lane has a primary
+    unsigned long long vmask;  // ballot(valid)
+};
+// Whether this lane's pair with secondary s (index j) survives; *m = the wave's
+// survivor mask. `live` (wave-uniform) masks secondaries past the window.
+template <int MODE, bool SAME>
+__device__ __forceinline__ bool qpass(const QFilter& f, uint2 s, int j, bool live, unsigned long long* m)
+{
+    const short2v d = __builtin_bit_cast(short2v, s.x) - __builtin_bit_cast(short2v, f.qxy);
+    bool p;
+    unsigned long long mm;
+    if (MODE == MODE_RPPI || MODE == MODE_WP) {
+        const int r = __builtin_amdgcn_sdot2(d, d, f.qlo, false);
+        const bool a = (unsigned)r < f.qrange, b = (unsigned)((int)s.y + f.zoff) < f.zspan;
+        p = a & b;
+        mm = __builtin_amdgcn_ballot_w64(a) & __builtin_amdgcn_ballot_w64(b);
+    } else {
+        const int dz = (int)s.y - f.qz;
+        const int r = __builtin_amdgcn_sdot2(d, d, __mul24(dz, dz) + f.qlo, false);
+        p = (unsigned)r < f.qrange;
+        mm = __builtin_amdgcn_ballot_w64(p);
+    }
+    if (SAME) {
+        const bool g = j > f.ilim;
+        p = p & g;
+        mm &= __builtin_amdgcn_ballot_w64(g);
+    } else {
+        p = p & f.valid;
+        mm &= f.vmask;
+    }
+    *m = live ? mm : 0ull;
+    return p & live;
+}
+
+// Appends this lane (if p) to the ring at tail, entry (j << 6) | lane; m = the
+// wave's mask of such lanes.
+__device__ __forceinline__ void enqueue(unsigned* qe, bool p, unsigned long long m, unsigned& tail, unsigned entry)
+{
+    if (p) {
+        const int b = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+        qe[(tail + (unsigned)b) & (kQueue - 1)] = entry;
+    }
+    tail += (unsigned)__popcll(m);
+}
+
+// Filters secondaries [jb, jhi) kBlockJ at a time and enqueues the survivors,
+// until the end or until 64 survivors are waiting; returns the next jb. The
+// block at jb has been loaded into s0..s3; each block loads the next one ahead
+// (q is padded by kQPad entries, so the look-ahead stays inside the array).
+template <int MODE, bool SAME>
+__device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jhi, uint2& s0, uint2& s1, uint2& s2,
+                                     uint2& s3, unsigned* qe, unsigned head, unsigned& tail)
+{
+    const unsigned lane = (unsigned)lane_id();
+    while (jb < jhi && tail - head < 64) {
+        const uint2 c0 = s0, c1 = s1, c2 = s2, c3 = s3;
+        const int jn = jb + kBlockJ;
+        KUint* pn = qs + 2 * (int64_t)jn;
+        s0 = make_uint2(pn[0], pn[1]);
+        s1 = make_uint2(pn[2], pn[3]);
+        s2 = make_uint2(pn[4], pn[5]);
+        s3 = make_uint2(pn[6], pn[7]);
+        const int nleft = jhi - jb;
+        unsigned long long m0, m1, m2, m3;
+        const bool p0 = qpass<MODE, SAME>(f, c0, jb, true, &m0);
+        const bool p1 = qpass<MODE, SAME>(f, c1, jb + 1, nleft > 1, &m1);
+        const bool p2 = qpass<MODE, SAME>(f, c2, jb + 2, nleft > 2, &m2);
+        const bool p3 = qpass<MODE, SAME>(f, c3, jb + 3, nleft > 3, &m3);
+        if (m0 | m1 | m2 | m3) {
+            const unsigned e0 = ((unsigned)jb << 6) | lane;
+            enqueue(qe, p0, m0, tail, e0);
+            enqueue(qe, p1, m1, tail, e0 + 64);
+            enqueue(qe, p2, m2, tail, e0 + 128);
+            enqueue(qe, p3, m3, tail, e0 + 192);
+        }
+        jb = jn;
+    }
+    return jb;
+}
+
 // One work item of the tuned kernel (see the comment above kQueue).
 template <int MODE, int R2F>
 __device__ __forceinline__ void process_item(const PairParams& P, const DevLattice& L1, const DevLattice& L2,
@@ -760,7 +868,6 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
                                              double4* prim)
 {
     const int lane = lane_id();
-    const float hi2 = P.hi2_f, lo2 = P.lo2_f, zr = P.zreach_f;
     const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
     const int flags = __builtin_amdgcn_readfirstlane(it.flags);
     const int dense = P.dense_mode == 1 ? 1 : (P.dense_mode == 2 ? 0 : (flags & kItemDense));
@@ -770,6 +877,9 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     const int i = i0 + lane * (stride > 0 ? stride : 1);
     const int ivalid = i < i1;
     const int c2 = cp.c2;
+    const int same = cp.same;
+    // a lane takes secondary j iff j > ilim (its primary exists; same cell: j > i)
+    const int ilim = !ivalid ? INT_MAX : (same ? i : INT_MIN);
     double xpos = 0, ypos = 0, zpos = 0, wi = 0;
     if (ivalid) {
         const double4 pi = L1.d4[i];
@@ -797,7 +907,7 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
             jhi = L2.bstart[cb + shi + 1];
         }
     }
-    if (cp.same && jlo < i0 + 1) jlo = i0 + 1;
+    if (same && jlo < i0 + 1) jlo = i0 + 1;
     jlo = __builtin_amdgcn_readfirstlane(jlo);
     jhi = __builtin_amdgcn_readfirstlane(jhi);
     {
@@ -812,90 +922,62 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     }
 
     if (jhi <= jlo) return;
-    const int jlast = jhi - 1;
 
     if (dense) {
-        const int same = cp.same;
-        const double4* __restrict__ d4 = L2.d4;
-        double4 cur = d4[min(jlo + lane, jlast)];
-        for (int jb = jlo; jb < jhi; jb += 64) {
-            const double4 nxt = d4[min(jb + 64 + lane, jlast)];
-            const int m = min(64, jhi - jb);
-            for (int k = 0; k < m; k++) {
-                const int j = jb + k;
-                const double xj = bcast_d(cur.x, k), yj = bcast_d(cur.y, k), zj = bcast_d(cur.z, k);
-                const double wj = P.need_w ? bcast_d(cur.w, k) : 0.0;
-                exact_pair<MODE, R2F>(P, cp, L1, i, ivalid && (!same || j > i), xpos, ypos, zpos, wi, xj, yj, zj,
-                                      wj, S);
-            }
+        // FP64 lockstep; the secondary record is a scalar load (d4 is padded, so
+        // the look-ahead load of j + 1 stays inside the array)
+        KDouble* ds = (KDouble*)(const void*)L2.d4;
+        double4 cur = kload_d4(ds, jlo);
+        for (int j = jlo; j < jhi; j++) {
+            const double4 nxt = kload_d4(ds, j + 1);
+            exact_pair<MODE, R2F>(P, cp, L1, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
+                                  P.need_w ? cur.w : 0.0, S);
             cur = nxt;
         }
         return;
     }
 
+    // sparse: quantize the wave's primaries relative to the secondary cell
     const int ixy2 = c2 / L2.nz, iy2 = ixy2 % L2.ny, ix2 = ixy2 / L2.ny;
     const double ox = L2.xmin + ix2 * L2.xbs, oy = L2.ymin + iy2 * L2.ybs, oz = L2.zmin + iz2 * L2.zbs;
-    const float xr = (float)(xpos - ox), yr = (float)(ypos - oy), zr_i = (float)(zpos - oz);
+    const uint2 qi = quantize_pos(xpos - ox, ypos - oy, zpos - oz, L2.qinv);
+    QFilter f;
+    f.qxy = qi.x;
+    f.qz = (int)qi.y;
+    f.qlo = L2.qlo_neg;
+    f.qrange = L2.qrange;
+    f.zoff = L2.qz - 1 - f.qz;
+    f.zspan = 2u * (unsigned)L2.qz - 1u;
+    f.ilim = ilim;
+    f.valid = ivalid;
+    f.vmask = __builtin_amdgcn_ballot_w64(ivalid);
     const int istride = stride > 0 ? stride : 1;
-    int head = 0, tail = 0, count = 0;
-    const int same = cp.same;
+    unsigned head = 0, tail = 0;
     // the wave's primaries (wrapped, FP64) for the drains
     if (P.prim_cache) {
         prim[lane] = make_double4(xpos, ypos, zpos, wi);
         __builtin_amdgcn_wave_barrier();
     }
-    const float4* __restrict__ f4 = L2.f4;
-    float4 cur = f4[min(jlo + lane, jlast)];
-    for (int jb = jlo; jb < jhi; jb += 64) {
-        const float4 nxt = f4[min(jb + 64 + lane, jlast)];
-        const int mt = min(64, jhi - jb);
-        // two secondaries per step: j0 = jb + k, j1 = jb + k + 1 (valid when k + 1 < mt)
-        for (int k = 0; k < mt; k += 2) {
-            const int k1 = min(k + 1, 63);
-            const int j0 = jb + k, j1 = jb + k + 1;
-            const float dx0 = bcast_f(cur.x, k) - xr, dy0 = bcast_f(cur.y, k) - yr, dz0 = bcast_f(cur.z, k) - zr_i;
-            const float dx1 = bcast_f(cur.x, k1) - xr, dy1 = bcast_f(cur.y, k1) - yr, dz1 = bcast_f(cur.z, k1) - zr_i;
-            int p0, p1;
-            if (MODE == MODE_RPPI || MODE == MODE_WP) {
-                const float q0 = fmaf(dy0, dy0, dx0 * dx0), q1 = fmaf(dy1, dy1, dx1 * dx1);
-                p0 = q0 < hi2 && q0 >= lo2 && fabsf(dz0) < zr;
-                p1 = q1 < hi2 && q1 >= lo2 && fabsf(dz1) < zr;
-            } else {
-                const float q0 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
-                const float q1 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
-                p0 = q0 < hi2 && q0 >= lo2;
-                p1 = q1 < hi2 && q1 >= lo2;
-            }
-            p0 = p0 && ivalid && (!same || j0 > i);
-            p1 = p1 && ivalid && (k + 1 < mt) && (!same || j1 > i);
-            const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
-            if (m0 | m1) {
-                // branch-free enqueue: survivors take consecutive ring entries,
-                // the other lanes write their scratch entry
-                const int b0 = __builtin_amdgcn_mbcnt_hi((unsigned)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m0, 0));
-                const int b1 = __builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m1, 0));
-                const int n0 = __popcll(m0);
-                const int pos0 = p0 ? ((tail + b0) & (kQueue - 1)) : (kQueue + lane);
-                const int pos1 = p1 ? ((tail + n0 + b1) & (kQueue - 1)) : (kQueue + lane);
-                qe[pos0] = ((unsigned)j0 << 6) | (unsigned)lane;
-                qe[pos1] = ((unsigned)j1 << 6) | (unsigned)lane;
-                const int added = n0 + __popcll(m1);
-                tail += added;
-                count += added;
-                while (count >= 64) {
-                    __builtin_amdgcn_wave_barrier();
-                    drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, 64, S);
-                    __builtin_amdgcn_wave_barrier();
-                    head += 64;
-                    count -= 64;
-                }
-            }
+    KUint* qs = (KUint*)(const void*)L2.q;
+    KUint* p0 = qs + 2 * (int64_t)jlo;
+    uint2 s0 = make_uint2(p0[0], p0[1]), s1 = make_uint2(p0[2], p0[3]);
+    uint2 s2 = make_uint2(p0[4], p0[5]), s3 = make_uint2(p0[6], p0[7]);
+    int jb = jlo;
+    while (jb < jhi) {
+        if (same)
+            jb = qscan<MODE, true>(f, qs, jb, jhi, s0, s1, s2, s3, qe, head, tail);
+        else
+            jb = qscan<MODE, false>(f, qs, jb, jhi, s0, s1, s2, s3, qe, head, tail);
+        while (tail - head >= 64) {
+            __builtin_amdgcn_wave_barrier();
+            drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, 64, S);
+            __builtin_amdgcn_wave_barrier();
+            head += 64;
         }
-        cur = nxt;
     }
-    if (count > 0) {
+    if (tail != head) {
         __builtin_amdgcn_wave_barrier();
-        drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, count, S);
+        drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, (int)(tail - head), S);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -960,7 +1042,7 @@ __global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLatt
     int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
     S.e = e;
     S.lut = lut;
-    unsigned* qe = reinterpret_cast<unsigned*>(qbase) + wave * kQueueAlloc;
+    unsigned* qe = reinterpret_cast<unsigned*>(qbase) + __builtin_amdgcn_readfirstlane(wave) * kQueueAlloc;
     // primaries cache: 16-byte aligned after the queues
     const uintptr_t pa = (reinterpret_cast<uintptr_t>(qbase + nwaves * kQueueAlloc) + 15) & ~(uintptr_t)15;
     double4* prim = reinterpret_cast<double4*>(pa) + wave * 64;
