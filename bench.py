@@ -47,9 +47,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X published FP32 vector rate
-FLOPS_PER_PAIR = {"countpairs": 8, "countpairs_xi": 8, "countpairs_s_mu": 8,   # 3 sub, 3 mul, 2 add
-                  "countpairs_rp_pi": 6, "countpairs_wp": 6}                    # 3 sub, 2 mul, 1 add
+# The pair filter is 32-bit integer vector work: 256 CUs x 64 lanes x 2.4 GHz =
+# 39.3e12 lane-instructions/s, at most 4 ops each (a packed dot product: 2 mul +
+# 2 add), i.e. 157.3 TOP/s.
+PEAK_VALU_TOPS = 157.3
+# integer ops of the pair filter per tested pair: 3-D: 3 sub, 3 mul, 2 add, the
+# lower-bound offset add, 1 compare; projected: 3 sub, 2 mul, 1 add, the offset
+# add, the z offset add, 2 compares
+OPS_PER_PAIR = {"countpairs": 10, "countpairs_xi": 10, "countpairs_s_mu": 10,
+                "countpairs_rp_pi": 10, "countpairs_wp": 10}
 METRIC = "pairs tested/s on 1 MI355X: tuned vs plain GPU form, npairs bit-exact vs FP64"
 
 
@@ -476,7 +482,7 @@ def main():
     dom = [s for s in st_t[0]][0]
     k_ms = float(np.mean([s[0]["kernel_ms"] for s in st_t]))
     dom_pairs = dom["tested_pairs"]
-    achieved = dom_pairs * FLOPS_PER_PAIR[wl.dominant] / (k_ms * 1e-3) / 1e12
+    achieved = dom_pairs * OPS_PER_PAIR[wl.dominant] / (k_ms * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_k_pairs_tuned_%s.txt" % wl.name)
     if os.path.isfile(tpath):
@@ -512,7 +518,8 @@ def main():
                 ms.append((time.perf_counter() - t0) * 1e3)
                 ks.append(st[0]["kernel_ms"])
             same = all(np.array_equal(a["npairs"], b["npairs"]) for a, b in zip(outs, out_t))
-            sweep[v] = {"ms_per_step": float(np.median(ms)), "kernel_ms": float(np.median(ks)), "npairs_equal": same}
+            sweep[v] = {"ms_per_step": float(np.median(ms)), "kernel_ms": float(np.median(ks)), "npairs_equal": same,
+                        "hist_copies": st[0]["hist_copies"]}
             print("sweep %s=%s: %.2f ms/step, kernel %.3f ms" % (var, v, sweep[v]["ms_per_step"],
                                                                 sweep[v]["kernel_ms"]), file=sys.stderr, flush=True)
         if old is None:
@@ -545,16 +552,17 @@ def main():
         "config": {"workload": wl.description, "tested_pairs_per_step": tested, "parallelism": "single GPU",
                    "scheduler": args.sched, "autotune": bool(args.autotune),
                    "engine_tested_pairs_per_step": engine_tested,
-                   "refine": list(st_t[-1][0]["refine"]), "autotuned": st_t[-1][0]["autotuned"]},
+                   "refine": list(st_t[-1][0]["refine"]), "autotuned": st_t[-1][0]["autotuned"],
+                   "hist_copies": st_t[-1][0]["hist_copies"]},
         "plain_value": plain,
         "tuned_over_plain": (value / plain) if plain else None,
         "useful_pairs_per_s": useful,
         "parity": {"tuned_vs_plain_npairs_equal": parity_plain, "gpu_vs_cpu_reference_npairs_equal": parity_cpu},
         "roofline": {"bound": "valu", "kernel": "k_pairs_tuned<%s>" % wl.dominant, "achieved": achieved,
-                     "peak": PEAK_FP32_VECTOR_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VECTOR_TFLOPS,
+                     "peak": PEAK_VALU_TOPS, "unit": "TOP/s", "frac": achieved / PEAK_VALU_TOPS,
                      "traffic": traffic, "kernel_ms": k_ms,
-                     "algorithmic": "%d FP32 flops per tested pair x %.4g tested pairs per launch"
-                                    % (FLOPS_PER_PAIR[wl.dominant], dom_pairs)},
+                     "algorithmic": "%d integer ops per tested pair x %.4g tested pairs per launch"
+                                    % (OPS_PER_PAIR[wl.dominant], dom_pairs)},
         "cpu_baseline": cpu,
     }
     if ab is not None:

@@ -708,8 +708,6 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
                         double* g_avg, double* g_w, bool timed, TunedRun& T)
 {
     Engine& E = eng();
-    const int threads = 256, nwaves = threads / 64;
-    const int regions = batch ? nwaves : 1;
     const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
     const void* kfn = nullptr;
 #define CF_K(M, F, B) if (P.mode == M && P.r2form == F && batch == B) kfn = (const void*)&k_pairs_tuned<M, F, B>;
@@ -724,45 +722,71 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
                 batch ? " (batched)" : "");
         return EXIT_FAILURE;
     }
-    const size_t lds_fixed = (size_t)nbin * 8 + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nwaves * kQueueAlloc * 4;
-    const size_t lds_prim = 16 + (size_t)nwaves * 64 * sizeof(double4);
-    auto lds_for = [&](int r) { return (size_t)regions * nslots * per_slot * r + lds_fixed; };
-    // resident workgroups per CU of this kernel (registers and LDS), which is
-    // also how many persistent workgroups per CU are launched
-    auto resident = [&](size_t l) {
-        int b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kfn, threads, l) != hipSuccess) {
-            (void)hipGetLastError();
-            b = (int)((160 * 1024) / (l + 1024));
-        }
-        return b < 1 ? 1 : (b > 8 ? 8 : b);
+    // Launch shape for a workgroup of `thr` threads: LDS histogram replication
+    // R (power of 2, <= 32, histograms <= 40 KiB) = the largest that keeps
+    // min(16, best achievable) waves per CU resident; the primaries cache where
+    // it costs no resident workgroup; `per_cu` = resident workgroups per CU
+    // (registers and LDS), which is also how many persistent workgroups per CU
+    // are launched.
+    struct Shape {
+        int threads, nwaves, R, prim, per_cu;
+        size_t lds;
     };
-    // LDS histogram replication R (power of 2, <= 32, histograms <= 40 KiB): the
-    // largest that keeps min(4, best achievable) workgroups per CU resident
-    const int target = std::min(4, resident(lds_for(1)));
-    int R = 1;
-    for (int r = 2; r <= 32; r *= 2) {
-        if ((size_t)regions * nslots * per_slot * r > 40 * 1024) break;
-        if (resident(lds_for(r)) < target) break;
-        R = r;
+    auto shape_for = [&](int thr) {
+        Shape sh;
+        sh.threads = thr;
+        sh.nwaves = thr / 64;
+        const int regions = batch ? sh.nwaves : 1;
+        const size_t lds_fixed =
+            (size_t)nbin * 8 + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)sh.nwaves * kQueueAlloc * 4;
+        const size_t lds_prim = 16 + (size_t)sh.nwaves * 64 * sizeof(double4);
+        auto lds_for = [&](int r) { return (size_t)regions * nslots * per_slot * r + lds_fixed; };
+        auto resident = [&](size_t l) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kfn, thr, l) != hipSuccess) {
+                (void)hipGetLastError();
+                nb = (int)((160 * 1024) / (l + 1024));
+            }
+            return nb < 1 ? 1 : (nb > 8 ? 8 : nb);
+        };
+        const int target = std::min(16, resident(lds_for(1)) * sh.nwaves);
+        int R = 1;
+        for (int r = 2; r <= 32; r *= 2) {
+            if ((size_t)regions * nslots * per_slot * r > 40 * 1024) break;
+            if (resident(lds_for(r)) * sh.nwaves < target) break;
+            R = r;
+        }
+        if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
+            const int v = atoi(rc);
+            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
+        }
+        if (E.force_R > 0 && E.force_R < R) R = E.force_R;
+        sh.R = R;
+        const size_t lds_base = lds_for(R);
+        sh.prim = resident(lds_base + lds_prim) >= resident(lds_base) ? 1 : 0;
+        if (const char* pc = getenv("CORRFUNC_AMD_PRIM_CACHE")) sh.prim = atoi(pc) != 0;
+        sh.lds = lds_base + (sh.prim ? lds_prim : 0);
+        sh.per_cu = resident(sh.lds);
+        return sh;
+    };
+    // workgroups of 8 waves (a histogram shared by more waves; measured as fast
+    // or faster), or of 4 where 8 would keep fewer waves resident or exceed LDS
+    Shape sh = shape_for(256);
+    {
+        const Shape s8 = shape_for(512);
+        if (s8.lds <= 64 * 1024 && s8.per_cu * s8.nwaves >= sh.per_cu * sh.nwaves) sh = s8;
+        if (const char* wg = getenv("CORRFUNC_AMD_THREADS"))
+            sh = atoi(wg) == 512 && s8.lds <= 64 * 1024 ? s8 : shape_for(256);
     }
-    if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
-        const int v = atoi(rc);
-        if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
-    }
-    if (E.force_R > 0 && E.force_R < R) R = E.force_R;
-    P.hist_copies = R;
-    const size_t lds_base = lds_for(R);
-    const int res_base = resident(lds_base);
-    // the primaries cache only where it costs no resident workgroup
-    P.prim_cache = resident(lds_base + lds_prim) >= res_base ? 1 : 0;
-    if (const char* pc = getenv("CORRFUNC_AMD_PRIM_CACHE")) P.prim_cache = atoi(pc) != 0;
-    const size_t lds = lds_base + (P.prim_cache ? lds_prim : 0);
+    const int threads = sh.threads, nwaves = sh.nwaves;
+    P.hist_copies = sh.R;
+    P.prim_cache = sh.prim;
+    const size_t lds = sh.lds;
     if (lds > 64 * 1024) {
         fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
         return EXIT_FAILURE;
     }
-    const int per_cu = resident(lds);
+    const int per_cu = sh.per_cu;
     const long max_blocks = (long)E.ncu * per_cu;
 
     // ---- cost prediction and order

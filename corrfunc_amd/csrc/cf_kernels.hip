@@ -611,7 +611,9 @@ constexpr int kBlockJ = 4;          // secondaries per scalar load of the sparse
 // Per-wave queue storage. Before a block of kBlockJ secondaries is enqueued the
 // ring holds < 64 entries, after it < 64 + 64 kBlockJ.
 constexpr int kQueueAlloc = kQueue;
-static_assert(64 + 64 * kBlockJ <= kQueue, "survivor ring too small for one block");
+// enqueue: the ring holds <= 63 + 64 t live entries when secondary t of a block
+// is enqueued, and the lanes without a survivor write up to 127 slots past them
+static_assert(64 * kBlockJ + 128 <= kQueue, "survivor ring too small for one block");
 // Secondary indices must fit the packed queue entry ((j << 6) | lane, 32 bits).
 constexpr int64_t kMaxSecondary = (int64_t)1 << 26;
 
@@ -781,84 +783,102 @@ struct QFilter {
     int qz, qlo, zoff;
     unsigned qrange, zspan;
     int ilim;                  // same cell: the lane takes j > ilim
-    bool valid;                // the lane has a primary
     unsigned long long vmask;  // ballot(valid)
 };
-// Whether this lane's pair with secondary s (index j) survives; *m = the wave's
-// survivor mask. `live` (wave-uniform) masks secondaries past the window.
+// The wave's mask of lanes whose pair with secondary s (index j) survives.
 template <int MODE, bool SAME>
-__device__ __forceinline__ bool qpass(const QFilter& f, uint2 s, int j, bool live, unsigned long long* m)
+__device__ __forceinline__ unsigned long long qpass(const QFilter& f, uint2 s, int j)
 {
     const short2v d = __builtin_bit_cast(short2v, s.x) - __builtin_bit_cast(short2v, f.qxy);
-    bool p;
-    unsigned long long mm;
+    unsigned long long m;
     if (MODE == MODE_RPPI || MODE == MODE_WP) {
         const int r = __builtin_amdgcn_sdot2(d, d, f.qlo, false);
-        const bool a = (unsigned)r < f.qrange, b = (unsigned)((int)s.y + f.zoff) < f.zspan;
-        p = a & b;
-        mm = __builtin_amdgcn_ballot_w64(a) & __builtin_amdgcn_ballot_w64(b);
+        m = __builtin_amdgcn_ballot_w64((unsigned)r < f.qrange) &
+            __builtin_amdgcn_ballot_w64((unsigned)((int)s.y + f.zoff) < f.zspan);
     } else {
         const int dz = (int)s.y - f.qz;
         const int r = __builtin_amdgcn_sdot2(d, d, __mul24(dz, dz) + f.qlo, false);
-        p = (unsigned)r < f.qrange;
-        mm = __builtin_amdgcn_ballot_w64(p);
+        m = __builtin_amdgcn_ballot_w64((unsigned)r < f.qrange);
     }
-    if (SAME) {
-        const bool g = j > f.ilim;
-        p = p & g;
-        mm &= __builtin_amdgcn_ballot_w64(g);
-    } else {
-        p = p & f.valid;
-        mm &= f.vmask;
-    }
-    *m = live ? mm : 0ull;
-    return p & live;
+    // same cell: the lane's own j > i; otherwise the lanes with a primary
+    return m & (SAME ? __builtin_amdgcn_ballot_w64(j > f.ilim) : f.vmask);
 }
 
-// Appends this lane (if p) to the ring at tail, entry (j << 6) | lane; m = the
-// wave's mask of such lanes.
-__device__ __forceinline__ void enqueue(unsigned* qe, bool p, unsigned long long m, unsigned& tail, unsigned entry)
+// Per-lane select by a lane mask: the lane's bit of m set ? a : b (one
+// v_cndmask; a C select here tends to become a branch around the computation of a).
+__device__ __forceinline__ unsigned lane_select(unsigned long long m, unsigned a, unsigned b)
 {
-    if (p) {
-        const int b = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-        qe[(tail + (unsigned)b) & (kQueue - 1)] = entry;
-    }
+    unsigned r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+    return r;
+}
+
+// Appends the lanes of m (nonzero) to the ring at tail with entry (j << 6) | lane.
+// Branch-free: the other lanes write slot tail + 64 + lane, past this
+// secondary's survivors and outside the live entries (kQueue >= 64 kBlockJ +
+// 128), where a later survivor overwrites it or it stays beyond the tail.
+__device__ __forceinline__ void enqueue(unsigned* qe, unsigned long long m, unsigned& tail, unsigned entry)
+{
+    const unsigned lane = (unsigned)lane_id();
+    const unsigned b =
+        (unsigned)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+    qe[(tail + lane_select(m, b, 64u + lane)) & (kQueue - 1)] = entry;
     tail += (unsigned)__popcll(m);
 }
 
-// Filters secondaries [jb, jhi) kBlockJ at a time and enqueues the survivors,
-// until the end or until 64 survivors are waiting; returns the next jb. The
-// block at jb has been loaded into s0..s3; each block loads the next one ahead
-// (q is padded by kQPad entries, so the look-ahead stays inside the array).
-template <int MODE, bool SAME>
-__device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jhi, uint2& s0, uint2& s1, uint2& s2,
-                                     uint2& s3, unsigned* qe, unsigned head, unsigned& tail)
+// A block of kBlockJ quantized secondaries in scalar registers.
+struct QBlock {
+    uint2 s[kBlockJ];
+};
+__device__ __forceinline__ QBlock qload(KUint* qs, int jb)
 {
-    const unsigned lane = (unsigned)lane_id();
-    while (jb < jhi && tail - head < 64) {
-        const uint2 c0 = s0, c1 = s1, c2 = s2, c3 = s3;
-        const int jn = jb + kBlockJ;
-        KUint* pn = qs + 2 * (int64_t)jn;
-        s0 = make_uint2(pn[0], pn[1]);
-        s1 = make_uint2(pn[2], pn[3]);
-        s2 = make_uint2(pn[4], pn[5]);
-        s3 = make_uint2(pn[6], pn[7]);
-        const int nleft = jhi - jb;
-        unsigned long long m0, m1, m2, m3;
-        const bool p0 = qpass<MODE, SAME>(f, c0, jb, true, &m0);
-        const bool p1 = qpass<MODE, SAME>(f, c1, jb + 1, nleft > 1, &m1);
-        const bool p2 = qpass<MODE, SAME>(f, c2, jb + 2, nleft > 2, &m2);
-        const bool p3 = qpass<MODE, SAME>(f, c3, jb + 3, nleft > 3, &m3);
-        if (m0 | m1 | m2 | m3) {
-            const unsigned e0 = ((unsigned)jb << 6) | lane;
-            enqueue(qe, p0, m0, tail, e0);
-            enqueue(qe, p1, m1, tail, e0 + 64);
-            enqueue(qe, p2, m2, tail, e0 + 128);
-            enqueue(qe, p3, m3, tail, e0 + 192);
-        }
-        jb = jn;
+    KUint* p = qs + 2 * (int64_t)jb;
+    QBlock b;
+#pragma unroll
+    for (int t = 0; t < kBlockJ; t++) b.s[t] = make_uint2(p[2 * t], p[2 * t + 1]);
+    return b;
+}
+
+// Filters block b (secondaries jb .. jb + kBlockJ - 1, of which the first
+// `nlive` exist) and enqueues the survivors.
+template <int MODE, bool SAME, bool PARTIAL>
+__device__ __forceinline__ void qblock(const QFilter& f, const QBlock& b, int jb, int nlive, unsigned* qe,
+                                       unsigned& tail)
+{
+    unsigned long long m[kBlockJ];
+#pragma unroll
+    for (int t = 0; t < kBlockJ; t++) {
+        m[t] = qpass<MODE, SAME>(f, b.s[t], jb + t);
+        if (PARTIAL) m[t] = t < nlive ? m[t] : 0ull;
     }
-    return jb;
+    const unsigned e0 = ((unsigned)jb << 6) | (unsigned)lane_id();
+#pragma unroll
+    for (int t = 0; t < kBlockJ; t++)
+        if (m[t]) enqueue(qe, m[t], tail, e0 + 64u * t);
+}
+
+// Filters the full blocks from jb on and enqueues their survivors, until fewer
+// than kBlockJ secondaries are left or 64 survivors are waiting; returns the next
+// jb. On entry and exit `a` holds the block at jb: the loop alternates between
+// two register blocks so that each block is loaded one block ahead (q is padded
+// by kQPad entries, so the look-ahead stays inside the array).
+template <int MODE, bool SAME>
+__device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jhi, QBlock& a, unsigned* qe,
+                                     unsigned head, unsigned& tail)
+{
+    for (;;) {
+        if (jb + kBlockJ > jhi || tail - head >= 64) return jb;
+        const QBlock b = qload(qs, jb + kBlockJ);
+        qblock<MODE, SAME, false>(f, a, jb, kBlockJ, qe, tail);
+        jb += kBlockJ;
+        if (jb + kBlockJ > jhi || tail - head >= 64) {
+            a = b;
+            return jb;
+        }
+        a = qload(qs, jb + kBlockJ);
+        qblock<MODE, SAME, false>(f, b, jb, kBlockJ, qe, tail);
+        jb += kBlockJ;
+    }
 }
 
 // One work item of the tuned kernel (see the comment above kQueue).
@@ -949,7 +969,6 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     f.zoff = L2.qz - 1 - f.qz;
     f.zspan = 2u * (unsigned)L2.qz - 1u;
     f.ilim = ilim;
-    f.valid = ivalid;
     f.vmask = __builtin_amdgcn_ballot_w64(ivalid);
     const int istride = stride > 0 ? stride : 1;
     unsigned head = 0, tail = 0;
@@ -959,15 +978,21 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         __builtin_amdgcn_wave_barrier();
     }
     KUint* qs = (KUint*)(const void*)L2.q;
-    KUint* p0 = qs + 2 * (int64_t)jlo;
-    uint2 s0 = make_uint2(p0[0], p0[1]), s1 = make_uint2(p0[2], p0[3]);
-    uint2 s2 = make_uint2(p0[4], p0[5]), s3 = make_uint2(p0[6], p0[7]);
+    QBlock a = qload(qs, jlo);
     int jb = jlo;
     while (jb < jhi) {
         if (same)
-            jb = qscan<MODE, true>(f, qs, jb, jhi, s0, s1, s2, s3, qe, head, tail);
+            jb = qscan<MODE, true>(f, qs, jb, jhi, a, qe, head, tail);
         else
-            jb = qscan<MODE, false>(f, qs, jb, jhi, s0, s1, s2, s3, qe, head, tail);
+            jb = qscan<MODE, false>(f, qs, jb, jhi, a, qe, head, tail);
+        if (jb < jhi && jb + kBlockJ > jhi && tail - head < 64) {
+            // the last, partial block
+            if (same)
+                qblock<MODE, true, true>(f, a, jb, jhi - jb, qe, tail);
+            else
+                qblock<MODE, false, true>(f, a, jb, jhi - jb, qe, tail);
+            jb = jhi;
+        }
         while (tail - head >= 64) {
             __builtin_amdgcn_wave_barrier();
             drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, 64, S);
@@ -1010,7 +1035,7 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
 // catalog it is working on, and the histogram of catalog k lives at
 // g_*[k*nslots ...].
 template <int MODE, int R2F, bool BATCH>
-__global__ __launch_bounds__(256) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
+__global__ __launch_bounds__(512) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
                                                      const WorkItem* __restrict__ items,
                                                      const int* __restrict__ nitems_p, int* __restrict__ counter,
