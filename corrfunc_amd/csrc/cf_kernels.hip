@@ -604,16 +604,16 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 //    cannot be in range; the survivors are compacted into a per-wave LDS queue
 //    and decided 64 at a time in FP64.
 // Either way every counted pair is decided by the reference's FP64 arithmetic.
-constexpr int kQueue = 512;         // survivor ring per wave (entries (j << 6) | lane)
 constexpr int kQueues = 8;          // work sub-queues (one counter each)
 constexpr int kCounterStride = 32;  // ints between sub-queue counters (128 B)
 constexpr int kBlockJ = 4;          // secondaries per scalar load of the sparse path
-// Per-wave queue storage. Before a block of kBlockJ secondaries is enqueued the
-// ring holds < 64 entries, after it < 64 + 64 kBlockJ.
-constexpr int kQueueAlloc = kQueue;
-// enqueue: the ring holds <= 63 + 64 t live entries when secondary t of a block
-// is enqueued, and the lanes without a survivor write up to 127 slots past them
-static_assert(64 * kBlockJ + 128 <= kQueue, "survivor ring too small for one block");
+// Survivor buffer per wave: entries (j << 6) | lane at [head, tail). A scan runs
+// while fewer than 64 entries wait (head < kCompactAt), a block adds <= 64
+// kBlockJ, and lanes without a survivor write up to 127 slots past the tail
+// (see enqueue); drains take 64 from the head; once head >= kCompactAt the
+// remaining < 64 entries move to the front.
+constexpr int kCompactAt = 128;
+constexpr int kQueueAlloc = kCompactAt + 64 + 64 * (kBlockJ - 1) + 127;
 // Secondary indices must fit the packed queue entry ((j << 6) | lane, 32 bits).
 constexpr int64_t kMaxSecondary = (int64_t)1 << 26;
 
@@ -681,11 +681,29 @@ __device__ __forceinline__ double sep2(double dx, double dy, double dz)
     return (dx * dx + dy * dy) + dz * dz;
 }
 
+// The same pointer, opaque to the compiler: loads through it are issued where
+// they are written (not hoisted, not merged with earlier ones), which keeps
+// values that only rare paths need out of the registers of the hot loops.
+template <typename T>
+__device__ __forceinline__ const T* fresh(const T* p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+// The same for read-only data seen through the constant address space, so the
+// loads are scalar.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* fresh_k(const T* p)
+{
+    asm volatile("" : "+s"(p));
+    return (const __attribute__((address_space(4))) T*)(const void*)p;
+}
+
 // Decides one (i, j) pair in FP64 exactly as the reference does and adds it to
 // the LDS histogram. (xpos, ypos, zpos) = primary position plus the cell pair's
 // wrap; (xj, yj, zj) = secondary position. `ok` masks lanes without a pair.
 template <int MODE, int R2F>
-__device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRec& cp, const DevLattice& L1, int i,
+__device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRec* cpp, const DevLattice* l1p, int i,
                                            int ok, double xpos, double ypos, double zpos, double wi, double xj,
                                            double yj, double zj, double wj, KernelLds& S)
 {
@@ -703,10 +721,13 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
     // pimax; only such pairs replay them.
     const bool near = r2 >= P.sqr_rmax - P.near_r2 || (MODE != MODE_DD && MODE != MODE_XI && adz >= P.pimax - P.near_dz);
     if (near) {
+        // the cell pair record and the primary set, loaded here
+        const CellPairRec* c = fresh(cpp);
+        const DevLattice* l1 = fresh(l1p);
         ITerms t;
-        i_terms(P, MODE, cp, L1.x[i], L1.y[i], L1.z[i], &t);
+        i_terms(P, MODE, *c, l1->x[i], l1->y[i], l1->z[i], &t);
         if (is2d) {
-            if (!cp.same && (t.skip || zj <= t.target)) return;
+            if (!c->same && (t.skip || zj <= t.target)) return;
         } else {
             if (t.skip || zj <= t.target || dz >= t.max_dz) return;
         }
@@ -750,26 +771,27 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
 }
 
 template <int MODE, int R2F>
-__device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const DevLattice& L1,
-                                            const DevLattice& L2, int i0, int istride, const unsigned* qe,
+__device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const CellPairRec* cpp,
+                                            const DevLattice* l1p, const double4* d4_1, const double4* d4_2,
+                                            const double3& wrap, int i0, int istride, const unsigned* qe,
                                             const double4* prim, unsigned head, int ndrain, KernelLds& S)
 {
     const int lane = lane_id();
     if (lane < ndrain) {
-        const unsigned e = qe[(head + lane) & (kQueue - 1)];
+        const unsigned e = qe[head + lane];
         const int j = (int)(e >> 6), l = (int)(e & 63);
         const int i = i0 + l * istride;
         double4 pi;
         if (P.prim_cache) {
             pi = prim[l];  // the primary's wrapped position and weight
         } else {
-            pi = L1.d4[i];
-            pi.x += cp.wx * P.xwrap;
-            pi.y += cp.wy * P.ywrap;
-            pi.z += cp.wz * P.zwrap;
+            pi = d4_1[i];
+            pi.x += wrap.x;
+            pi.y += wrap.y;
+            pi.z += wrap.z;
         }
-        const double4 pj = L2.d4[j];
-        exact_pair<MODE, R2F>(P, cp, L1, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
+        const double4 pj = d4_2[j];
+        exact_pair<MODE, R2F>(P, cpp, l1p, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
 }
 
@@ -783,7 +805,6 @@ struct QFilter {
     int qz, qlo, zoff;
     unsigned qrange, zspan;
     int ilim;                  // same cell: the lane takes j > ilim
-    unsigned long long vmask;  // ballot(valid)
 };
 // The wave's mask of lanes whose pair with secondary s (index j) survives.
 template <int MODE, bool SAME>
@@ -800,8 +821,9 @@ __device__ __forceinline__ unsigned long long qpass(const QFilter& f, uint2 s, i
         const int r = __builtin_amdgcn_sdot2(d, d, __mul24(dz, dz) + f.qlo, false);
         m = __builtin_amdgcn_ballot_w64((unsigned)r < f.qrange);
     }
-    // same cell: the lane's own j > i; otherwise the lanes with a primary
-    return m & (SAME ? __builtin_amdgcn_ballot_w64(j > f.ilim) : f.vmask);
+    // same cell: the lane's own j > i (lanes without a primary have quantized
+    // positions no secondary passes)
+    return SAME ? m & __builtin_amdgcn_ballot_w64(j > f.ilim) : m;
 }
 
 // Per-lane select by a lane mask: the lane's bit of m set ? a : b (one
@@ -813,16 +835,17 @@ __device__ __forceinline__ unsigned lane_select(unsigned long long m, unsigned a
     return r;
 }
 
-// Appends the lanes of m (nonzero) to the ring at tail with entry (j << 6) | lane.
-// Branch-free: the other lanes write slot tail + 64 + lane, past this
-// secondary's survivors and outside the live entries (kQueue >= 64 kBlockJ +
-// 128), where a later survivor overwrites it or it stays beyond the tail.
+// Appends the lanes of m (nonzero) to the survivor buffer at tail with entry
+// (j << 6) | lane. Branch-free: the other lanes write slot tail + 64 + lane, past
+// this secondary's survivors, where a later survivor overwrites it or it stays
+// beyond the tail.
 __device__ __forceinline__ void enqueue(unsigned* qe, unsigned long long m, unsigned& tail, unsigned entry)
 {
     const unsigned lane = (unsigned)lane_id();
     const unsigned b =
         (unsigned)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-    qe[(tail + lane_select(m, b, 64u + lane)) & (kQueue - 1)] = entry;
+    unsigned* qt = qe + __builtin_amdgcn_readfirstlane(tail);
+    qt[lane_select(m, b, 64u + lane)] = entry;
     tail += (unsigned)__popcll(m);
 }
 
@@ -866,31 +889,38 @@ template <int MODE, bool SAME>
 __device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jhi, QBlock& a, unsigned* qe,
                                      unsigned head, unsigned& tail)
 {
+    int nfull = (jhi - jb) / kBlockJ;
     for (;;) {
-        if (jb + kBlockJ > jhi || tail - head >= 64) return jb;
+        if (nfull <= 0 || tail - head >= 64) return jb;
         const QBlock b = qload(qs, jb + kBlockJ);
         qblock<MODE, SAME, false>(f, a, jb, kBlockJ, qe, tail);
         jb += kBlockJ;
-        if (jb + kBlockJ > jhi || tail - head >= 64) {
+        if (--nfull <= 0 || tail - head >= 64) {
             a = b;
             return jb;
         }
         a = qload(qs, jb + kBlockJ);
         qblock<MODE, SAME, false>(f, b, jb, kBlockJ, qe, tail);
         jb += kBlockJ;
+        --nfull;
     }
 }
 
-// One work item of the tuned kernel (see the comment above kQueue).
+// One work item of the tuned kernel (see the comment above kQueues).
 template <int MODE, int R2F>
-__device__ __forceinline__ void process_item(const PairParams& P, const DevLattice& L1, const DevLattice& L2,
-                                             const CellPairRec& cp, const WorkItem& it, KernelLds& S, unsigned* qe,
-                                             double4* prim)
+__device__ __forceinline__ void process_item(const PairParams& P, const DevLattice* l1p, const DevLattice* l2p,
+                                             const CellPairRec& cp, const CellPairRec* cpp, const WorkItem& it,
+                                             KernelLds& S, unsigned* qe, double4* prim)
 {
+    // the lattices and the set-up parameters are loaded per item (fresh), so
+    // that they do not occupy registers across the persistent loop
+    const __attribute__((address_space(4))) DevLattice& L1 = *fresh_k(l1p);
+    const __attribute__((address_space(4))) DevLattice& L2 = *fresh_k(l2p);
+    const PairParams& Pi = P;
     const int lane = lane_id();
     const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
     const int flags = __builtin_amdgcn_readfirstlane(it.flags);
-    const int dense = P.dense_mode == 1 ? 1 : (P.dense_mode == 2 ? 0 : (flags & kItemDense));
+    const int dense = Pi.dense_mode == 1 ? 1 : (Pi.dense_mode == 2 ? 0 : (flags & kItemDense));
     // dense items take strided primaries (whole-cell sample per wave), which
     // spreads the lanes' pairs over more histogram slots
     const int stride = flags >> kStrideShift;
@@ -900,18 +930,21 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     const int same = cp.same;
     // a lane takes secondary j iff j > ilim (its primary exists; same cell: j > i)
     const int ilim = !ivalid ? INT_MAX : (same ? i : INT_MIN);
+    const double3 wrap = make_double3(cp.wx * Pi.xwrap, cp.wy * Pi.ywrap, cp.wz * Pi.zwrap);
+    const double4* d4_1 = L1.d4;
+    const double4* d4_2 = L2.d4;
     double xpos = 0, ypos = 0, zpos = 0, wi = 0;
     if (ivalid) {
-        const double4 pi = L1.d4[i];
-        xpos = pi.x + cp.wx * P.xwrap;
-        ypos = pi.y + cp.wy * P.ywrap;
-        zpos = pi.z + cp.wz * P.zwrap;
+        const double4 pi = d4_1[i];
+        xpos = pi.x + wrap.x;
+        ypos = pi.y + wrap.y;
+        zpos = pi.z + wrap.z;
         wi = pi.w;
     }
     // secondary z-slabs this chunk can reach
     const int iz2 = c2 % L2.nz;
-    const double zlo = wave_min_d(ivalid ? zpos : DBL_MAX) - P.zreach - P.slab_margin;
-    const double zhi = wave_max_d(ivalid ? zpos : -DBL_MAX) + P.zreach + P.slab_margin;
+    const double zlo = wave_min_d(ivalid ? zpos : DBL_MAX) - Pi.zreach - Pi.slab_margin;
+    const double zhi = wave_max_d(ivalid ? zpos : -DBL_MAX) + Pi.zreach + Pi.slab_margin;
     int jlo, jhi;
     {
         const int64_t cb = (int64_t)c2 * kSlabs;
@@ -946,11 +979,11 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     if (dense) {
         // FP64 lockstep; the secondary record is a scalar load (d4 is padded, so
         // the look-ahead load of j + 1 stays inside the array)
-        KDouble* ds = (KDouble*)(const void*)L2.d4;
+        KDouble* ds = (KDouble*)(const void*)d4_2;
         double4 cur = kload_d4(ds, jlo);
         for (int j = jlo; j < jhi; j++) {
             const double4 nxt = kload_d4(ds, j + 1);
-            exact_pair<MODE, R2F>(P, cp, L1, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
+            exact_pair<MODE, R2F>(P, cpp, l1p, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
                                   P.need_w ? cur.w : 0.0, S);
             cur = nxt;
         }
@@ -960,7 +993,12 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     // sparse: quantize the wave's primaries relative to the secondary cell
     const int ixy2 = c2 / L2.nz, iy2 = ixy2 % L2.ny, ix2 = ixy2 / L2.ny;
     const double ox = L2.xmin + ix2 * L2.xbs, oy = L2.ymin + iy2 * L2.ybs, oz = L2.zmin + iz2 * L2.zbs;
-    const uint2 qi = quantize_pos(xpos - ox, ypos - oy, zpos - oz, L2.qinv);
+    // a lane without a primary gets (qx, qy) = (-32768, -32768): every difference
+    // to a secondary (0 <= q <= kQMax) wraps to |dq| >= 32768 - kQMax per axis, so
+    // its squared distance exceeds any threshold (which is < (kQMax + 2)^2 + 1)
+    // without overflowing 32 bits
+    uint2 qi = quantize_pos(xpos - ox, ypos - oy, zpos - oz, L2.qinv);
+    if (!ivalid) qi = make_uint2(0x80008000u, 0u);
     QFilter f;
     f.qxy = qi.x;
     f.qz = (int)qi.y;
@@ -969,7 +1007,6 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     f.zoff = L2.qz - 1 - f.qz;
     f.zspan = 2u * (unsigned)L2.qz - 1u;
     f.ilim = ilim;
-    f.vmask = __builtin_amdgcn_ballot_w64(ivalid);
     const int istride = stride > 0 ? stride : 1;
     unsigned head = 0, tail = 0;
     // the wave's primaries (wrapped, FP64) for the drains
@@ -995,14 +1032,26 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         }
         while (tail - head >= 64) {
             __builtin_amdgcn_wave_barrier();
-            drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, 64, S);
+            drain_queue<MODE, R2F>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head, 64, S);
             __builtin_amdgcn_wave_barrier();
             head += 64;
+        }
+        if (head >= kCompactAt) {
+            // move the < 64 waiting entries to the front
+            const unsigned n = tail - head;
+            unsigned v = 0;
+            if ((unsigned)lane < n) v = qe[head + lane];
+            __builtin_amdgcn_wave_barrier();
+            if ((unsigned)lane < n) qe[lane] = v;
+            __builtin_amdgcn_wave_barrier();
+            head = 0;
+            tail = n;
         }
     }
     if (tail != head) {
         __builtin_amdgcn_wave_barrier();
-        drain_queue<MODE, R2F>(P, cp, L1, L2, i0, istride, qe, prim, head, (int)(tail - head), S);
+        drain_queue<MODE, R2F>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head, (int)(tail - head),
+                               S);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -1035,7 +1084,7 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
 // catalog it is working on, and the histogram of catalog k lives at
 // g_*[k*nslots ...].
 template <int MODE, int R2F, bool BATCH>
-__global__ __launch_bounds__(512) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
                                                      const WorkItem* __restrict__ items,
                                                      const int* __restrict__ nitems_p, int* __restrict__ counter,
@@ -1069,8 +1118,10 @@ __global__ __launch_bounds__(512) void k_pairs_tuned(PairParams P, const DevLatt
     S.lut = lut;
     unsigned* qe = reinterpret_cast<unsigned*>(qbase) + __builtin_amdgcn_readfirstlane(wave) * kQueueAlloc;
     // primaries cache: 16-byte aligned after the queues
-    const uintptr_t pa = (reinterpret_cast<uintptr_t>(qbase + nwaves * kQueueAlloc) + 15) & ~(uintptr_t)15;
-    double4* prim = reinterpret_cast<double4*>(pa) + wave * 64;
+    // (offsets from the LDS base, so the pointer stays an LDS pointer)
+    const size_t poff = ((size_t)(reinterpret_cast<char*>(qbase + nwaves * kQueueAlloc) -
+                                  reinterpret_cast<char*>(smem_u64)) + 15) & ~(size_t)15;
+    double4* prim = reinterpret_cast<double4*>(reinterpret_cast<char*>(smem_u64) + poff) + wave * 64;
     for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
     for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
     for (int s = threadIdx.x; s < nreg; s += blockDim.x) {
@@ -1080,7 +1131,8 @@ __global__ __launch_bounds__(512) void k_pairs_tuned(PairParams P, const DevLatt
     }
     __syncthreads();
     const int nitems = *nitems_p;
-    DevLattice L1 = lats[0], L2 = lats[BATCH ? 0 : 1];
+    const DevLattice* l1p = lats;
+    const DevLattice* l2p = lats + (BATCH ? 0 : 1);
     int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
     // Work queue: item g belongs to sub-queue g % kQueues (the cost order is kept
     // inside every sub-queue); sub-queue q has its own counter at
@@ -1122,7 +1174,8 @@ __global__ __launch_bounds__(512) void k_pairs_tuned(PairParams P, const DevLatt
         next += kQueues;
         last = item;
         const WorkItem it = items[item];
-        const CellPairRec cp = cps[__builtin_amdgcn_readfirstlane(it.cp)];
+        const CellPairRec* cpp = cps + __builtin_amdgcn_readfirstlane(it.cp);
+        const CellPairRec cp = *cpp;
         if (BATCH) {
             const int k = __builtin_amdgcn_readfirstlane(cp.pad);
             if (k != cur) {
@@ -1132,15 +1185,15 @@ __global__ __launch_bounds__(512) void k_pairs_tuned(PairParams P, const DevLatt
                     __builtin_amdgcn_wave_barrier();
                 }
                 cur = k;
-                L1 = lats[k];
-                L2 = L1;
+                l1p = lats + k;
+                l2p = l1p;
             }
         }
         // per-cell-pair cost record: wall-clock ticks of the wave on this item
         // (branch-free in the lanes: lane 0 adds dt, the others 0; every lane
         // stores the same compute-unit id)
         const unsigned long long t0 = cp_ticks ? wall_clock64() : 0ull;
-        process_item<MODE, R2F>(P, L1, L2, cp, it, S, qe, prim);
+        process_item<MODE, R2F>(P, l1p, l2p, cp, cpp, it, S, qe, prim);
         if (cp_ticks) {
             const unsigned long long dt = wall_clock64() - t0;
             atomicAdd(&cp_ticks[it.cp], lane == 0 ? dt : 0ull);
