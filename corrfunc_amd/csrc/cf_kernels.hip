@@ -351,11 +351,15 @@ __device__ int neighbour_added(const NgbArgs& a, int64_t icell, int k, const int
     if (!eval_neighbour(a, icell, k, bs1, bs2, bd1, bd2, rec)) return 0;
     if (a.dupcheck) {
         // the reference skips a neighbour already appended for this primary cell with
-        // the same secondary cell and the same wrap (gridlink_utils.h.src:46-72)
-        CellPairRec prev;
+        // the same secondary cell and the same wrap (gridlink_utils.h.src:46-72).
+        // prev is initialized: eval_neighbour writes it only when it returns 1, and a
+        // compiler may evaluate the comparisons below on an unwritten record and
+        // combine them with the result (observed: neighbours dropped depending on
+        // leftover register contents)
+        CellPairRec prev = {};
         for (int kk = 0; kk < k; kk++) {
-            if (eval_neighbour(a, icell, kk, bs1, bs2, bd1, bd2, &prev) && prev.c2 == rec->c2 &&
-                prev.wx == rec->wx && prev.wy == rec->wy && prev.wz == rec->wz)
+            const int added = eval_neighbour(a, icell, kk, bs1, bs2, bd1, bd2, &prev);
+            if (added && prev.c2 == rec->c2 && prev.wx == rec->wx && prev.wy == rec->wy && prev.wz == rec->wz)
                 return 0;
         }
     }
@@ -368,7 +372,7 @@ __global__ void k_cellpair_flag(NgbArgs a, int64_t ncells, int nngb, const int* 
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= ncells * nngb) return;
-    CellPairRec r;
+    CellPairRec r = {};
     flags[t] = neighbour_added(a, t / nngb, (int)(t % nngb), bs1, bs2, bd1, bd2, &r);
 }
 
@@ -380,7 +384,7 @@ __global__ void k_cellpair_emit(NgbArgs a, int64_t ncells, int nngb, const int* 
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     double my = 0;
     if (t < ncells * nngb) {
-        CellPairRec r;
+        CellPairRec r = {};
         if (neighbour_added(a, t / nngb, (int)(t % nngb), bs1, bs2, bd1, bd2, &r)) {
             r.pad = cat;
             out[pos[t]] = r;
@@ -573,7 +577,7 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
     const int i0 = L1.bstart[(int64_t)cp.c1 * kSlabs], i1 = L1.bstart[(int64_t)(cp.c1 + 1) * kSlabs];
     const int j0 = L2.bstart[(int64_t)cp.c2 * kSlabs], j1 = L2.bstart[(int64_t)(cp.c2 + 1) * kSlabs];
     for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        ITerms t;
+        ITerms t = {};
         i_terms(P, mode, cp, L1.x[i], L1.y[i], L1.z[i], &t);
         if (t.skip) continue;
         const double wi = P.need_w ? L1.w[i] : 0.0;
@@ -724,7 +728,7 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
         // the cell pair record and the primary set, loaded here
         const CellPairRec* c = fresh(cpp);
         const DevLattice* l1 = fresh(l1p);
-        ITerms t;
+        ITerms t = {};
         i_terms(P, MODE, *c, l1->x[i], l1->y[i], l1->z[i], &t);
         if (is2d) {
             if (!c->same && (t.skip || zj <= t.target)) return;

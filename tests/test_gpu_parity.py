@@ -176,3 +176,53 @@ def test_engine_variables_keep_counts(engine, oracle, knobs, mode, sched):
         scheduler.set_scheduler("learned")
     want = run(oracle, mode, "oracle_", options=_opts(**o), **kw)
     assert_same(mode, got, want, what=str(knobs))
+
+
+def _lattice(n_side, spacing, offset, jitter, seed):
+    """Points on a cubic lattice (pairs exactly at multiples of the spacing),
+    optionally displaced by a tiny jitter, shifted by `offset`."""
+    g = np.arange(n_side, dtype=np.float64) * spacing
+    X = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1) + offset
+    if jitter:
+        X = X + np.random.default_rng(seed).uniform(-jitter, jitter, X.shape)
+    return X
+
+
+# (name, points, bins, box, periodic, pimax, max_cells, refine): each case puts
+# many pairs at the quantized filter's thresholds or stretches its quantum
+FILTER_CASES = [
+    # pairs exactly at rmax, at the bin edges and at pimax (spacing 2.5)
+    ("edges_exact", lambda: _lattice(16, 2.5, 0.0, 0.0, 0), np.arange(1, 8) * 2.5, 40.0, True, 5.0, 100, (2, 2, 1)),
+    # the same, a few ulps off the edges, far from the origin (non-periodic)
+    ("edges_jitter_far", lambda: _lattice(16, 2.5, 1.0e4, 1e-12 * 1e4, 1), np.arange(1, 9) * 2.5, None, False,
+     5.0, 100, (2, 2, 1)),
+    # lower bound far from zero: only a thin shell counts
+    ("thin_shell", lambda: uniform(20000, 100.0, 3), np.array([9.0, 9.5, 10.0]), 100.0, True, 10.0, 100,
+     (2, 2, 1)),
+    # cells much larger than rmax (coarse quantum)
+    ("big_cells", lambda: uniform(20000, 100.0, 4), np.logspace(-1.5, np.log10(0.8), 6), 100.0, True, 3.0, 4,
+     (1, 1, 1)),
+    # rmax close to half the box, cells smaller than rmax
+    ("small_cells", lambda: uniform(6000, 50.0, 5), np.linspace(1.0, 24.0, 7), 50.0, True, 24.0, 100, (3, 3, 3)),
+]
+
+
+@pytest.mark.parametrize("eng", ["tuned", "sparse"])
+@pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_s_mu"])
+@pytest.mark.parametrize("case", FILTER_CASES, ids=[c[0] for c in FILTER_CASES])
+def test_filter_boundaries(engine, oracle, eng, mode, case):
+    """npairs bit-exact where pairs sit on the separation thresholds of the
+    integer pair filter (rmax, rmin, pimax) and where its quantum is coarse."""
+    _select(eng)
+    name, make, bins, box, periodic, pimax, max_cells, refine = case
+    X = np.asarray(make())
+    n = X.shape[1]
+    W = weights(n, 7, np.float64)
+    kw = mode_kwargs(mode, X, W, autocorr=1, bins=bins, L=box, pimax=pimax, mu_max=1.0, nmu_bins=5)
+    o = dict(periodic=periodic, boxsize=box if periodic else None, need_avg_sep=True, isa="avx512f",
+             max_cells_per_dim=max_cells, refine=refine)
+    got = run(engine, mode, options=_opts(**o), **kw)
+    want = run(oracle, mode, "oracle_", options=_opts(**o), **kw)
+    assert_same(mode, got, want, what=name)
+    if name == "edges_exact":
+        assert want["npairs"].sum() > 0
