@@ -426,7 +426,7 @@ static int cellpairs_flag(const DevLattice& L1, const DevLattice& L2, const int 
 static void cellpairs_emit(const CpPlan& cp, CellPairRec* out, double* dtested, int cat)
 {
     Engine& E = eng();
-    const unsigned blocks = (unsigned)((cp.total + 255) / 256);
+    const unsigned blocks = (unsigned)std::min<int64_t>((cp.total + 255) / 256, 1024);
     k_cellpair_emit<<<blocks, 256, 0, E.stream>>>(cp.a, cp.L1.ncells, cp.nngb, cp.L1.bstart, cp.L2.bstart,
                                                   cp.L1.bounds, cp.L2.bounds, cp.pos, out, dtested, cat);
     CF_CHECK(hipGetLastError());

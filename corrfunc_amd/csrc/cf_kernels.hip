@@ -62,16 +62,23 @@ __global__ void k_minmax(int64_t n, const T* __restrict__ X, const T* __restrict
         for (int d = 0; d < 6; d++) partial[blockIdx.x * 6 + d] = s[d][0];
 }
 
+// One wave: the lanes reduce strided subsets of the block partials, then the
+// wave (min/max are exact, so the order does not matter).
 __global__ void k_minmax_final(int nblocks, const double* __restrict__ partial, double* __restrict__ out)
 {
-    if (threadIdx.x != 0) return;
+    const int lane = (int)threadIdx.x;
     double r[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int b = 0; b < nblocks; b++)
+    for (int b = lane; b < nblocks; b += 64)
         for (int d = 0; d < 3; d++) {
             r[d] = fmin(r[d], partial[b * 6 + d]);
             r[3 + d] = fmax(r[3 + d], partial[b * 6 + 3 + d]);
         }
-    for (int d = 0; d < 6; d++) out[d] = r[d];
+    for (int d = 0; d < 3; d++) {
+        r[d] = wave_min_d(r[d]);
+        r[3 + d] = wave_max_d(r[3 + d]);
+    }
+    if (lane == 0)
+        for (int d = 0; d < 6; d++) out[d] = r[d];
 }
 
 // --------------------------------------------------------- gridding (a5)
@@ -376,24 +383,34 @@ __global__ void k_cellpair_flag(NgbArgs a, int64_t ncells, int nngb, const int* 
     flags[t] = neighbour_added(a, t / nngb, (int)(t % nngb), bs1, bs2, bd1, bd2, &r);
 }
 
+// Grid-stride over the candidates; the pairs tested are summed per workgroup
+// (integers below 2^53, exact in any order) and added with one atomic per
+// workgroup, as every atomic on the single counter is serialized.
 __global__ void k_cellpair_emit(NgbArgs a, int64_t ncells, int nngb, const int* __restrict__ bs1,
                                 const int* __restrict__ bs2, const double* __restrict__ bd1,
                                 const double* __restrict__ bd2, const int* __restrict__ pos,
                                 CellPairRec* __restrict__ out, double* __restrict__ tested, int cat)
 {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __shared__ double part[4];
+    const int64_t total = ncells * nngb;
     double my = 0;
-    if (t < ncells * nngb) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
         CellPairRec r = {};
         if (neighbour_added(a, t / nngb, (int)(t % nngb), bs1, bs2, bd1, bd2, &r)) {
             r.pad = cat;
             out[pos[t]] = r;
             const double n1 = (double)cell_count(bs1, r.c1), n2 = (double)cell_count(bs2, r.c2);
-            my = r.same ? n1 * (n1 - 1) * 0.5 : n1 * n2;
+            my += r.same ? n1 * (n1 - 1) * 0.5 : n1 * n2;
         }
     }
     for (int o = 32; o > 0; o >>= 1) my += __shfl_xor(my, o, 64);
-    if ((threadIdx.x & 63) == 0 && my != 0) atomicAdd(tested, my);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = my;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double s = part[0] + part[1] + part[2] + part[3];
+        if (s != 0) atomicAdd(tested, s);
+    }
 }
 
 // ------------------------------------------------- exact pair replay (a8-a12)
