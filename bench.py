@@ -28,8 +28,8 @@ inputs: gridding, work list, pair kernel, histogram reduction, finalisation):
 pair list (default options) of N_i * N_j (N(N-1)/2 for a cell with itself),
 reported by the engine for each call.
 
-The tuned form (FP32 filter + FP64 replay, persistent waves, cost-ordered
-work list) is the reported value; the plain FP64 GPU form runs the same steps
+The tuned form (integer filter on quantized positions + FP64 replay,
+persistent waves, cost-ordered work list) is the reported value; the plain FP64 GPU form runs the same steps
 for comparison (workload 3: one plain call per catalog). The CPU baseline is
 the reference library compiled from its own sources (oracle/_ref), run on the
 host cores on the workload (or a stated sample of it) in the FP64 path, whose
@@ -133,7 +133,7 @@ def npairs_equal(mode, a, b):
 class Workload:
     name = ""
     description = ""
-    dtype = "f32 filter + f64 decisions and sums"
+    dtype = "i16/i32 filter + f64 decisions and sums"
     data = ""
     dominant = ""        # mode of the dominant pair kernel
 

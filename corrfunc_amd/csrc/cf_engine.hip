@@ -560,8 +560,9 @@ static int choose_grid(int mode, int64_t N1, double rmax, double pimax, double b
     return EXIT_SUCCESS;
 }
 
-// Launch parameters of the tuned kernel: rounding order, cut values, edge
-// windows, FP32 filter margins and the bin lookup table.
+// Launch parameters of the pair kernels: rounding order, cut values, edge
+// windows, slab window and the bin lookup table (the integer filter's
+// thresholds depend on the secondary set's quantum: set_quant_filter).
 static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, const std::vector<double>& rupp_sqr,
                          int sorted, double rmax, double pimax, double inv_d2, double sqr_mu_max, int need_avg,
                          int need_w, int autocorr, const double wrap[3], double coord_scale, double cs, int rmaxr,

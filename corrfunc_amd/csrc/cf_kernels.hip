@@ -5,12 +5,13 @@
 //   k_bucket_count                cell index + z-slab per particle, bucket histogram
 //                                                          (gridlink_impl.c.src:162-201)
 //   k_scan_*                      exclusive prefix sum of the bucket histogram
-//   k_scatter                     counting-sort scatter into SoA arrays in HBM (FP64 + FP32 local)
+//   k_scatter                     counting-sort scatter into SoA arrays in HBM (FP64 + quantized
+//                                 cell-local positions for the integer pair filter)
 //   k_cell_bounds                 per-cell bounding boxes  (gridlink_impl.c.src:292-316)
 //   k_cellpair_flag / _emit       neighbour-cell work list with periodic wraps and
 //                                 min-separation culling   (gridlink_impl.c.src:439-625)
 //   k_pairs_plain                 FP64 pair kernel, one workgroup per cell pair
-//   k_pairs_tuned                 FP32 filter + FP64 replay, persistent waves, LDS histograms
+//   k_pairs_tuned                 integer filter + FP64 replay, persistent waves, LDS histograms
 //
 // All FP64 arithmetic that decides a pair's bin is written in the reference's
 // operation order; this file is compiled with -ffp-contract=off so the compiler

@@ -21,8 +21,8 @@
 extern "C" {
 #endif
 
-/* Pair-kernel variants. TUNED (default): FP32 filter on cell-local coordinates,
- * FP64 replay of the surviving pairs, persistent waves. PLAIN: every pair of
+/* Pair-kernel variants. TUNED (default): integer filter on quantized cell-local
+ * positions, FP64 replay of the surviving pairs, persistent waves. PLAIN: every pair of
  * every cell pair decided in FP64, one workgroup per cell pair. Both give the
  * same counts. The environment variable CORRFUNC_AMD_ENGINE=plain selects PLAIN
  * at load time. */
