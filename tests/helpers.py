@@ -1,4 +1,6 @@
 """Input generators and comparison helpers shared by the tests."""
+import struct
+
 import numpy as np
 
 from corrfunc_amd import _abi
@@ -78,3 +80,24 @@ def mode_kwargs(mode, X, W=None, X2=None, W2=None, autocorr=1, bins=None, L=None
     if mode == "countpairs_s_mu":
         kw.update(mu_max=mu_max, nmu_bins=nmu_bins)
     return kw
+
+
+def write_fastfood(path, cols, idat=None, fdat=None, znow=0.0, markers=None):
+    """A fast-food file: 5-int header record (second int = rows), a 9-float and
+    a 1-float record, one record per column; markers[k] overrides column k's
+    leading byte count (to plant corruption)."""
+    n = len(cols[0])
+    idat = list(idat) if idat is not None else [0, n, 0, 0, 0]
+    fdat = list(fdat) if fdat is not None else [0.0] * 9
+
+    def rec(f, payload, lead=None):
+        f.write(struct.pack("@i", len(payload) if lead is None else lead))
+        f.write(payload)
+        f.write(struct.pack("@i", len(payload)))
+
+    with open(path, "wb") as f:
+        rec(f, struct.pack("@5i", *idat))
+        rec(f, np.asarray(fdat, np.float32).tobytes())
+        rec(f, np.asarray([znow], np.float32).tobytes())
+        for k, c in enumerate(cols):
+            rec(f, np.ascontiguousarray(c).tobytes(), None if markers is None else markers.get(k))
