@@ -149,18 +149,45 @@ _SIGS = {
                       C.c_double, C.POINTER(results_countpairs_wp), C.POINTER(config_options),
                       C.POINTER(extra_options)],
 }
+class results_countpairs_mocks(C.Structure):
+    _fields_ = [("npairs", C.POINTER(C.c_uint64)), ("rupp", C.POINTER(C.c_double)),
+                ("rpavg", C.POINTER(C.c_double)), ("weightavg", C.POINTER(C.c_double)),
+                ("pimax", C.c_double), ("nbin", C.c_int), ("npibin", C.c_int)]
+
+
+class results_countpairs_mocks_s_mu(C.Structure):
+    _fields_ = [("npairs", C.POINTER(C.c_uint64)), ("supp", C.POINTER(C.c_double)),
+                ("savg", C.POINTER(C.c_double)), ("mu_max", C.c_double), ("mu_min", C.c_double),
+                ("weightavg", C.POINTER(C.c_double)), ("nsbin", C.c_int), ("nmu_bins", C.c_int)]
+
+
+_MOCK_SIGS = {
+    "countpairs_mocks": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                         C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int,
+                         C.POINTER(results_countpairs_mocks), C.POINTER(config_options),
+                         C.POINTER(extra_options)],
+    "countpairs_mocks_s_mu": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_int,
+                              C.POINTER(results_countpairs_mocks_s_mu), C.POINTER(config_options),
+                              C.POINTER(extra_options)],
+}
 _RES = {"countpairs": results_countpairs, "countpairs_rp_pi": results_countpairs_rp_pi,
         "countpairs_s_mu": results_countpairs_s_mu, "countpairs_xi": results_countpairs_xi,
-        "countpairs_wp": results_countpairs_wp}
+        "countpairs_wp": results_countpairs_wp, "countpairs_mocks": results_countpairs_mocks,
+        "countpairs_mocks_s_mu": results_countpairs_mocks_s_mu}
 _FREE = {"countpairs": "free_results", "countpairs_rp_pi": "free_results_rp_pi",
          "countpairs_s_mu": "free_results_s_mu", "countpairs_xi": "free_results_xi",
-         "countpairs_wp": "free_results_wp"}
-ENTRY_POINTS = tuple(_SIGS) + tuple(_FREE.values())
+         "countpairs_wp": "free_results_wp", "countpairs_mocks": "free_results_mocks",
+         "countpairs_mocks_s_mu": "free_results_mocks_s_mu"}
+ENTRY_POINTS = tuple(_SIGS) + tuple(_MOCK_SIGS) + tuple(_FREE.values())
 
 
 def bind(lib, prefix=""):
-    """Attach argtypes to the five entry points (and free functions) of `lib`."""
-    for name, sig in _SIGS.items():
+    """Attach argtypes to the entry points (and free functions) of `lib`: the
+    five theory counters and, where the library has them, the two on-sky ones."""
+    sigs = dict(_SIGS)
+    sigs.update({k: v for k, v in _MOCK_SIGS.items() if hasattr(lib, prefix + k)})
+    for name, sig in sigs.items():
         f = getattr(lib, prefix + name)
         f.argtypes = sig
         f.restype = C.c_int
@@ -406,5 +433,67 @@ def reported_slots(mode, out):
     n = out["nbin"]
     if mode in ("countpairs", "countpairs_xi", "countpairs_wp"):
         return np.arange(1, n)
-    m = out["npibin"] if mode == "countpairs_rp_pi" else out["nmu_bins"]
+    m = out["npibin"] if mode in ("countpairs_rp_pi", "countpairs_mocks") else out["nmu_bins"]
     return np.array([i * (m + 1) + j for i in range(1, n) for j in range(m)], dtype=np.int64)
+
+
+def call_mocks(lib, mode, prefix="", *, autocorr=1, nthreads=1, binfile=None, RA1=None, DEC1=None, CZ1=None,
+               weights1=None, RA2=None, DEC2=None, CZ2=None, weights2=None, weight_type=None, pimax=None,
+               mu_max=None, nmu_bins=None, cosmology=1, options=None):
+    """Run on-sky entry point `mode` ("countpairs_mocks" or "countpairs_mocks_s_mu")
+    of `lib`; returns a dict of numpy arrays as call_mode does, or None on failure.
+    The position arrays handed to C are copies (the library may fix them in place)."""
+    (RA1, DEC1, CZ1, RA2, DEC2, CZ2), dt = _prep_positions([RA1, DEC1, CZ1, RA2, DEC2, CZ2])
+    RA1, DEC1, CZ1 = [a.copy() for a in (RA1, DEC1, CZ1)]
+    if RA2 is not None:
+        RA2, DEC2, CZ2 = [a.copy() for a in (RA2, DEC2, CZ2)]
+    n1 = len(RA1)
+    n2 = len(RA2) if RA2 is not None else 0
+    method = WEIGHT_NONE
+    if weight_type is not None:
+        if weight_type not in ("pair_product", "p"):
+            raise ValueError("unknown weight_type %r" % weight_type)
+        method = WEIGHT_PAIR_PRODUCT
+    extra = get_extra_options(method)
+    if method != WEIGHT_NONE:
+        w1 = _prep_weights(weights1, n1, dt)
+        extra.weights0.weights[0] = w1.ctypes.data
+        if not autocorr:
+            w2 = _prep_weights(weights2, n2, dt)
+            extra.weights1.weights[0] = w2.ctypes.data
+    o = options if options is not None else make_options(periodic=False)
+    o.float_type = dt.itemsize
+    path, tmp = write_bin_file(binfile)
+    res = _RES[mode]()
+    fn = getattr(lib, prefix + mode)
+    try:
+        if mode == "countpairs_mocks":
+            st = fn(n1, _ptr(RA1), _ptr(DEC1), _ptr(CZ1), n2, _ptr(RA2), _ptr(DEC2), _ptr(CZ2), int(nthreads),
+                    int(autocorr), path.encode(), float(pimax), int(cosmology), C.byref(res), C.byref(o),
+                    C.byref(extra))
+        elif mode == "countpairs_mocks_s_mu":
+            st = fn(n1, _ptr(RA1), _ptr(DEC1), _ptr(CZ1), n2, _ptr(RA2), _ptr(DEC2), _ptr(CZ2), int(nthreads),
+                    int(autocorr), path.encode(), float(mu_max), int(nmu_bins), int(cosmology), C.byref(res),
+                    C.byref(o), C.byref(extra))
+        else:
+            raise ValueError(mode)
+    finally:
+        if tmp:
+            os.remove(path)
+    if st != 0:
+        return None
+    if mode == "countpairs_mocks":
+        n, m = res.nbin, res.npibin
+        tot = (n + 1) * (m + 1)
+        out = dict(nbin=n, npibin=m, pimax=res.pimax, npairs=_arr(res.npairs, tot, np.uint64),
+                   rupp=_arr(res.rupp, n, np.float64), ravg=_arr(res.rpavg, tot, np.float64),
+                   weightavg=_arr(res.weightavg, tot, np.float64))
+    else:
+        n, m = res.nsbin, res.nmu_bins
+        tot = (n + 1) * (m + 1)
+        out = dict(nbin=n, nmu_bins=m, mu_max=res.mu_max, npairs=_arr(res.npairs, tot, np.uint64),
+                   rupp=_arr(res.supp, n, np.float64), ravg=_arr(res.savg, tot, np.float64),
+                   weightavg=_arr(res.weightavg, tot, np.float64))
+    out["api_time"] = o.c_api_time
+    getattr(lib, prefix + _FREE[mode])(C.byref(res))
+    return out

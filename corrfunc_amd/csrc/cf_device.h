@@ -5,7 +5,9 @@
 
 namespace cfamd {
 
-enum Mode { MODE_DD = 0, MODE_RPPI = 1, MODE_SMU = 2, MODE_XI = 3, MODE_WP = 4 };
+// MODE_RPPI_MOCKS / MODE_SMU_MOCKS: on-sky catalogs (Cartesian positions from
+// RA, DEC, distance), (rp, pi) / (s, mu) split along the pair's mean line of sight.
+enum Mode { MODE_DD = 0, MODE_RPPI = 1, MODE_SMU = 2, MODE_XI = 3, MODE_WP = 4, MODE_RPPI_MOCKS = 5, MODE_SMU_MOCKS = 6 };
 
 // FP64 rounding orders of the reference per-ISA kernels.
 enum R2Form {
@@ -71,8 +73,9 @@ struct PairParams {
     int need_avg, need_w;
     int autocorr;
     int sorted_edges;  // rupp_sqr non-decreasing -> binary search
-    double sqr_rmax, sqr_rmin;
-    double pimax;      // DD/xi: rmax; rp_pi/wp: pimax; s_mu: smax*mu_max
+    double sqr_rmax, sqr_rmin; // on-sky modes: sqr_rmax is the 3-D limit (rpmax^2 + pimax^2, or smax^2)
+    double pimax;      // DD/xi: rmax; rp_pi/wp: pimax; s_mu: smax*mu_max; on-sky: pimax or mu_max
+    double sqr_rpmax;  // on-sky (rp, pi): rpmax^2
     double inv_d2, sqr_mu_max;
     double max_all_dz_same;  // same-cell z window (DD/xi/s_mu)
     double xwrap, ywrap, zwrap;

@@ -26,6 +26,7 @@
 #include "cf_model.hip"
 #include "../../include/corrfunc_theory.h"
 #include "../../include/corrfunc_amd.h"
+#include "../../include/corrfunc_mocks.h"
 
 namespace cfamd {
 
@@ -511,9 +512,32 @@ static int r2form_for(int mode, int isa_id)
 // variants: rp_pi_impl:277-329, s_mu_impl:276-340, xi_impl:212-270, wp_impl:240-290).
 // Updates o->bin_refine_factors as the reference does.
 static int choose_grid(int mode, int64_t N1, double rmax, double pimax, double boxsize, const double ext[6],
-                       const double wrap[3], struct config_options* o, GridSpec& g, int refine[3])
+                       const double wrap[3], struct config_options* o, GridSpec& g, int refine[3], int64_t N2 = 0)
 {
     const int is2d = mode == MODE_RPPI || mode == MODE_WP;
+    if (mode == MODE_RPPI_MOCKS || mode == MODE_SMU_MOCKS) {
+        // on-sky engines: rmax is the 3-D limit; refine 1 on an axis whose extent
+        // exceeds 20 rmax; boost with the larger set's density
+        // (countpairs_rp_pi_mocks_impl.c.src:404-477)
+        if (get_bin_refine_scheme(o) == BINNING_DFL)
+            for (int d = 0; d < 3; d++)
+                if (rmax < 0.05 * (ext[3 + d] - ext[d])) o->bin_refine_factors[d] = 1;
+        const double gsz[3] = {rmax, rmax, rmax};
+        for (int d = 0; d < 3; d++) refine[d] = o->bin_refine_factors[d];
+        if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+        const double ncell = (double)g.nx * g.ny * g.nz;
+        const double avg_np = (double)(N2 > N1 ? N2 : N1) / ncell;
+        const int max_nmesh = std::max(g.nx, std::max(g.ny, g.nz));
+        if ((max_nmesh <= 10 || avg_np >= 250) && max_nmesh < o->max_cells_per_dim &&
+            get_bin_refine_scheme(o) == BINNING_DFL) {
+            o->bin_refine_factors[0] += 1;
+            o->bin_refine_factors[1] += 1;
+            for (int d = 0; d < 3; d++) refine[d] = o->bin_refine_factors[d];
+            if (make_grid(g, ext, gsz, wrap, refine, o->max_cells_per_dim) != EXIT_SUCCESS) return EXIT_FAILURE;
+        }
+        g.qreach = rmax * (1.0 + 1e-6);
+        return EXIT_SUCCESS;
+    }
     if (get_bin_refine_scheme(o) == BINNING_DFL) {
         if (mode == MODE_DD) {
             for (int d = 0; d < 3; d++)
@@ -716,6 +740,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
     CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
     CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
+    CF_K(MODE_RPPI_MOCKS, R2_SUM_XY_Z, 0) CF_K(MODE_SMU_MOCKS, R2_SUM_XY_Z, 0)
     CF_K(MODE_DD, R2_SUM_XY_Z, 1) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 1)
 #undef CF_K
     if (!kfn) {
@@ -1024,6 +1049,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     const int mode = c.mode;
     const int is2d = mode == MODE_RPPI || mode == MODE_WP;
     const bool xiwp = mode == MODE_XI || mode == MODE_WP;
+    const bool onsky = mode == MODE_RPPI_MOCKS || mode == MODE_SMU_MOCKS;
     const size_t esize = o->float_type;
     const int need_w = extra->weight_method != NONE;
     R.need_w = need_w;
@@ -1039,13 +1065,13 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     double rmin = 0, rmax = 0;
     int nbin = 0;
     if (setup_bins(c.binfile, &rmin, &rmax, &nbin, R.rupp) != EXIT_SUCCESS) return EXIT_FAILURE;
-    if (!(rmin >= 0.0 && rmax > 0.0 && rmin < rmax && nbin > 0)) {
+    if (!((onsky ? rmin > 0.0 : rmin >= 0.0) && rmax > 0.0 && rmin < rmax && nbin > 0)) {
         fprintf(stderr, "Error: Could not setup with R bins correctly. (rmin = %lf, rmax = %lf, with nbins = %d). "
                         "Expected non-zero rmin/rmax with rmax > rmin and nbins >=1 \n", rmin, rmax, nbin);
         return EXIT_FAILURE;
     }
     R.nbin = nbin;
-    if (mode == MODE_SMU) {
+    if (mode == MODE_SMU || mode == MODE_SMU_MOCKS) {
         if (c.mu_max <= 0.0 || c.mu_max > 1.0) {
             fprintf(stderr, "Error: max_mu = %lf. Please change it to be > 0 and <= 1.0\n", c.mu_max);
             return EXIT_FAILURE;
@@ -1070,11 +1096,27 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         const double dmu = c.mu_max / n2;
         inv_d2 = 1.0 / dmu;
         sqr_mu_max = c.mu_max * c.mu_max;
+    } else if (mode == MODE_RPPI_MOCKS) {
+        // pi column width pimax / npibin; P.pimax and P.sqr_mu_max carry pimax, pimax^2
+        n2 = (int)c.pimax;
+        pimax = c.pimax;
+        const double dpi = pimax / n2;
+        inv_d2 = 1.0 / dpi;
+        sqr_mu_max = pimax * pimax;
+    } else if (mode == MODE_SMU_MOCKS) {
+        n2 = c.nmu_bins;
+        pimax = c.mu_max;
+        const double dmu = c.mu_max / (double)n2;
+        inv_d2 = 1.0 / dmu;
+        sqr_mu_max = c.mu_max * c.mu_max;
     } else {
         pimax = rmax;
     }
+    // on-sky engines: the largest 3-D separation a counted pair can have
+    const double max_sep = mode == MODE_RPPI_MOCKS ? sqrt(rmax * rmax + pimax * pimax) : rmax;
     R.n2 = n2;
-    const int64_t nslots = (mode == MODE_RPPI || mode == MODE_SMU) ? (int64_t)(n2 + 1) * (nbin + 1) : nbin;
+    const int64_t nslots =
+        (mode == MODE_RPPI || mode == MODE_SMU || onsky) ? (int64_t)(n2 + 1) * (nbin + 1) : nbin;
     std::vector<double> rupp_sqr(nbin);
     for (int i = 0; i < nbin; i++) rupp_sqr[i] = R.rupp[i] * R.rupp[i];
     int sorted = 1;
@@ -1113,7 +1155,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
                 ext[3 + d] = e2[3 + d] > ext[3 + d] ? e2[3 + d] : ext[3 + d];
             }
         }
-        if (o->periodic && o->boxsize == BOXSIZE_NOTGIVEN) {
+        if (!onsky && o->periodic && o->boxsize == BOXSIZE_NOTGIVEN) {
             fprintf(stderr, "boxsize = %g must be specified with periodic wrap. Please specify a non-zero boxsize, or "
                             "zero to detect the particle extent, or -1 to make a dimension non-periodic.\n", o->boxsize);
             return EXIT_FAILURE;
@@ -1126,6 +1168,10 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         wrap[0] = per[0] ? (o->boxsize_x > 0 ? o->boxsize_x : (ext[3] - ext[0])) : 0.;
         wrap[1] = per[1] ? (by > 0 ? by : (ext[4] - ext[1])) : 0.;
         wrap[2] = per[2] ? (bz > 0 ? bz : (ext[5] - ext[2])) : 0.;
+        if (onsky) {
+            per[0] = per[1] = per[2] = 0;
+            wrap[0] = wrap[1] = wrap[2] = 0.;
+        }
     }
 
     // ---- autotuned refine factors / histogram replication
@@ -1133,7 +1179,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         E.force_R = 0;
         E.force_fetch = 0;
         E.last_autotuned = 0;
-        if (E.autotune && get_bin_refine_scheme(o) == BINNING_DFL) {
+        if (E.autotune && get_bin_refine_scheme(o) == BINNING_DFL && !onsky) {
             std::array<int, 5> ch;
             const std::string key = tune_key(mode, c.N1, c.autocorr ? 0 : c.N2, c.autocorr, rmax, pimax, ext, wrap,
                                              need_w, o->need_avg_sep, esize, nbin);
@@ -1147,7 +1193,8 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     // ---- grid
     int refine[3];
     GridSpec g;
-    if (choose_grid(mode, c.N1, rmax, pimax, c.boxsize, ext, wrap, o, g, refine) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (choose_grid(mode, c.N1, onsky ? max_sep : rmax, pimax, c.boxsize, ext, wrap, o, g, refine, c.N2) != EXIT_SUCCESS)
+        return EXIT_FAILURE;
 
     // ---- gridding on the GPU
     DevLattice L1, L2;
@@ -1161,7 +1208,8 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
 
     // ---- cell pairs
     double max3 = -1, max2 = -1, max1 = -1;
-    if (mode == MODE_DD || mode == MODE_XI) max3 = rmax;
+    if (onsky) max3 = max_sep;
+    else if (mode == MODE_DD || mode == MODE_XI) max3 = rmax;
     else if (is2d) { max2 = rmax; max1 = pimax; }
     else { max3 = rmax; max1 = pimax; }
     CellPairRec* cps = nullptr;
@@ -1182,6 +1230,15 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
                                                 : (refine[1] > refine[2] ? refine[1] : refine[2]);
         build_params(mode, resolve_isa(o->instruction_set), nbin, n2, (int)nslots, rupp_sqr, sorted, rmax, pimax,
                      inv_d2, sqr_mu_max, o->need_avg_sep, need_w, c.autocorr, wrap, coord_scale, cs, rmaxr, P, lut);
+        if (onsky) {
+            // 3-D limit for the pruning, the filter and the slab window; rp^2 limit
+            // for the (rp, pi) bins (countpairs_rp_pi_mocks_impl.c.src:303-306)
+            P.sqr_rpmax = rupp_sqr[nbin - 1];
+            P.sqr_rmax = mode == MODE_RPPI_MOCKS ? rmax * rmax + pimax * pimax : rupp_sqr[nbin - 1];
+            P.max_all_dz_same = sqrt(P.sqr_rmax - 0.0 - 0.0);
+            P.zreach = sqrt(P.sqr_rmax);
+            P.near_r2 = 1e-9 * P.sqr_rmax + 1e-9 * max_sep * coord_scale;
+        }
         set_quant_filter(L1, P);
         set_quant_filter(L2, P);
     }
@@ -1728,6 +1785,208 @@ static int guarded(F&& f)
     } catch (const InputError&) {
         return EXIT_FAILURE;
     }
+}
+
+
+// ------------------------------------------------------------ on-sky inputs
+// (RA, DEC, cz or comoving distance) -> Cartesian FP64 on the host, with the
+// reference's steps and FP64 operation order (countpairs_rp_pi_mocks_impl.c.src:
+// 43-101 input fixes, 309-390 distances and positions; utils/set_cosmo_dist.c:
+// 25-71 the distance table; utils/cosmology_params.c:26-56 the cosmologies).
+constexpr double kSpeedOfLight = 299800.0;  // km/s (utils/set_cosmo_dist.h:19)
+constexpr double kPiOver180 = 0.017453292519943295769236907684886127134428718885417254560971;
+
+// The caller's arrays are fixed in place, in their own precision, as the reference does.
+template <typename T>
+static int sky_fix_inputs(int64_t n, T* ra, T* dec, T* cz, int is_comoving)
+{
+    if (n == 0) return EXIT_SUCCESS;
+    if (!ra || !dec || !cz) {
+        fprintf(stderr, "Input arrays can not be NULL. Have RA = %p DEC = %p cz = %p\n", (void*)ra, (void*)dec,
+                (void*)cz);
+        return EXIT_FAILURE;
+    }
+    bool fix_ra = false, fix_dec = false;
+    T max_cz = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (cz[i] > max_cz) max_cz = cz[i];
+        if (ra[i] < 0) fix_ra = true;
+        if (dec[i] > 90) fix_dec = true;
+        if (dec[i] > 180) {
+            fprintf(stderr, "theta[%lld] = %g should be less than 180 deg\n", (long long)i, (double)dec[i]);
+            return EXIT_FAILURE;
+        }
+    }
+    const bool fix_cz = max_cz < 10 && !is_comoving;
+    if (fix_ra) fprintf(stderr, "Out of range values found for ra. Expected ra in [0.0,360.0], found [-180,180] -- fixing that\n");
+    if (fix_dec) fprintf(stderr, "Out of range values found for dec. Expected dec in [-90.0,90.0], found [0,180] -- fixing that\n");
+    if (fix_cz) fprintf(stderr, "Out of range values found for cz. Expected `cz' but found `z' (max = %g) -- multiplying by c\n", (double)max_cz);
+    for (int64_t i = 0; i < n && (fix_ra || fix_dec || fix_cz); i++) {
+        if (fix_ra) ra[i] += (T)180.0;
+        if (fix_dec) dec[i] -= (T)90.0;
+        if (fix_cz) cz[i] *= (T)kSpeedOfLight;
+    }
+    return EXIT_SUCCESS;
+}
+
+static int sky_fix(int64_t n, void* ra, void* dec, void* cz, size_t esize, int is_comoving)
+{
+    return esize == 4 ? sky_fix_inputs(n, (float*)ra, (float*)dec, (float*)cz, is_comoving)
+                      : sky_fix_inputs(n, (double*)ra, (double*)dec, (double*)cz, is_comoving);
+}
+
+static int sky_cosmology(int which, double* om, double* ol)
+{
+    if (which == 1) *om = 0.25;        // LasDamas
+    else if (which == 2) *om = 0.302;  // Planck
+    else {
+        fprintf(stderr, "ERROR: Cosmology=%d not implemented\n", which);
+        return EXIT_FAILURE;
+    }
+    *ol = 1.0 - *om;
+    return EXIT_SUCCESS;
+}
+
+// Comoving distance (Mpc/h) at z = k/max_size, k = 1, 2, ..., up to zmax: Simpson's
+// rule over steps of 1e-2/max_size of 1/E(z), flat FRW.
+static int sky_distance_table(double zmax, int max_size, double om, double ol, std::vector<double>& zc,
+                              std::vector<double>& dc)
+{
+    zc.clear();
+    dc.clear();
+    const double ok = 1.0 - om - ol;
+    const double dh = kSpeedOfLight * 0.01 / 1.0;
+    const double step = 1.0 / max_size;
+    const double dz = 1e-2 * step;
+    double acc = 0.0, e2 = 1.0, znext = step;
+    for (double z = 2.0 * dz; z < zmax; z += 2.0 * dz) {
+        const double e0 = e2;
+        const double zm = 1 + z - dz, zh = 1 + z;
+        const double e1 = 1.0 / sqrt(om * (zm * zm * zm) + ok * zm + ol);
+        e2 = 1.0 / sqrt(om * (zh * zh * zh) + ok * zh + ol);
+        acc += dz * (e0 + 4. * e1 + e2) / 3.;
+        if (z > (znext - 1e-10) && z < (znext + 1e-10)) {
+            if ((int)zc.size() >= max_size) break;
+            zc.push_back(z);
+            dc.push_back(acc * dh);
+            znext += step;
+        }
+    }
+    return (int)zc.size();
+}
+
+// Piecewise-linear interpolation in the table; false outside it.
+static bool sky_interp(const std::vector<double>& xa, const std::vector<double>& ya, double x, double* y)
+{
+    const size_t n = xa.size();
+    if (n < 2 || !(x >= xa[0] && x <= xa[n - 1])) return false;
+    size_t lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+        const size_t m = (hi + lo) / 2;
+        if (xa[m] > x) hi = m;
+        else lo = m;
+    }
+    const double w = xa[lo + 1] - xa[lo];
+    *y = w > 0.0 ? ya[lo] + (x - xa[lo]) / w * (ya[lo + 1] - ya[lo]) : 0.0;
+    return true;
+}
+
+static double sky_value(const void* p, size_t esize, int64_t i)
+{
+    return esize == 4 ? (double)((const float*)p)[i] : ((const double*)p)[i];
+}
+
+struct SkySet {
+    std::vector<double> x, y, z, w;
+};
+
+// Both sets' Cartesian positions (and FP64 weights).
+static int sky_positions(int64_t n1, const void* ra1, const void* dec1, const void* cz1, const void* w1, int64_t n2,
+                         const void* ra2, const void* dec2, const void* cz2, const void* w2, int autocorr,
+                         size_t esize, int is_comoving, int cosmology, SkySet& s1, SkySet& s2)
+{
+    double om, ol;
+    if (sky_cosmology(cosmology, &om, &ol) != EXIT_SUCCESS) return EXIT_FAILURE;
+    std::vector<double> zc, dc;
+    const double inv_c = 1.0 / kSpeedOfLight;
+    if (!is_comoving) {
+        double czmax = 0.0;
+        for (int64_t i = 0; i < n1; i++)
+            if (sky_value(cz1, esize, i) > czmax) czmax = sky_value(cz1, esize, i);
+        if (!autocorr)
+            for (int64_t i = 0; i < n2; i++)
+                if (sky_value(cz2, esize, i) > czmax) czmax = sky_value(cz2, esize, i);
+        sky_distance_table(czmax * inv_c + 0.01, 10000, om, ol, zc, dc);
+    }
+    for (int set = 0; set < (autocorr ? 1 : 2); set++) {
+        const int64_t n = set ? n2 : n1;
+        const void *ra = set ? ra2 : ra1, *dec = set ? dec2 : dec1, *cz = set ? cz2 : cz1, *w = set ? w2 : w1;
+        SkySet& S = set ? s2 : s1;
+        S.x.resize(n);
+        S.y.resize(n);
+        S.z.resize(n);
+        for (int64_t i = 0; i < n; i++) {
+            double d = sky_value(cz, esize, i);
+            if (!is_comoving && !sky_interp(zc, dc, d * inv_c, &d)) {
+                fprintf(stderr, "Error: cz = %g is outside the tabulated redshift range [%g, %g] of the distance "
+                                "conversion\n", sky_value(cz, esize, i), zc.empty() ? 0.0 : zc.front() * kSpeedOfLight,
+                        zc.empty() ? 0.0 : zc.back() * kSpeedOfLight);
+                return EXIT_FAILURE;
+            }
+            const double de = sky_value(dec, esize, i), ra_i = sky_value(ra, esize, i);
+            const double cd = cos(de * kPiOver180);
+            S.x[i] = d * cd * cos(ra_i * kPiOver180);
+            S.y[i] = d * cd * sin(ra_i * kPiOver180);
+            S.z[i] = d * sin(de * kPiOver180);
+        }
+        if (w) {
+            S.w.resize(n);
+            for (int64_t i = 0; i < n; i++) S.w[i] = sky_value(w, esize, i);
+        }
+    }
+    return EXIT_SUCCESS;
+}
+
+// Shared driver of the two on-sky entry points: fixes, converts, counts, doubles
+// and averages (countpairs_rp_pi_mocks_impl.c.src:200-847, s_mu_mocks_impl:200-850).
+static int sky_run(int mode, int64_t ND1, void* RA1, void* DEC1, void* CZ1, int64_t ND2, void* RA2, void* DEC2,
+                   void* CZ2, int numthreads, int autocorr, const char* binfile, double pimax, double mu_max,
+                   int nmu_bins, int cosmology, struct config_options* o, struct extra_options* extra, CallOut& R)
+{
+    if (autocorr != 0 && autocorr != 1) {
+        fprintf(stderr, "Error: autocorr = %d must be 0 or 1\n", autocorr);
+        return EXIT_FAILURE;
+    }
+    const size_t esize = o->float_type;
+    if (sky_fix(ND1, RA1, DEC1, CZ1, esize, o->is_comoving_dist) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (!autocorr && sky_fix(ND2, RA2, DEC2, CZ2, esize, o->is_comoving_dist) != EXIT_SUCCESS) return EXIT_FAILURE;
+    double om, ol;
+    if (sky_cosmology(cosmology, &om, &ol) != EXIT_SUCCESS) return EXIT_FAILURE;
+    const int need_w = extra->weight_method != NONE;
+    SkySet s1, s2;
+    if (sky_positions(ND1, RA1, DEC1, CZ1, need_w ? extra->weights0.weights[0] : nullptr, ND2, RA2, DEC2, CZ2,
+                      need_w && !autocorr ? extra->weights1.weights[0] : nullptr, autocorr, esize,
+                      o->is_comoving_dist, cosmology, s1, s2) != EXIT_SUCCESS)
+        return EXIT_FAILURE;
+    if (need_w && (s1.w.size() != (size_t)ND1 || (!autocorr && s2.w.size() != (size_t)ND2))) {
+        fprintf(stderr, "Error: weight_method requires weights arrays\n");
+        return EXIT_FAILURE;
+    }
+    // the counting runs in FP64 on the converted positions
+    struct extra_options ex = *extra;
+    if (need_w) {
+        ex.weights0.weights[0] = s1.w.data();
+        ex.weights1.weights[0] = autocorr ? s1.w.data() : s2.w.data();
+    }
+    const size_t saved = o->float_type;
+    o->float_type = sizeof(double);
+    // N2 as given: the boost rule takes the larger of the two sets' densities
+    CallSpec c{mode, ND1, ND2, s1.x.data(), s1.y.data(), s1.z.data(),
+               autocorr ? nullptr : s2.x.data(), autocorr ? nullptr : s2.y.data(), autocorr ? nullptr : s2.z.data(),
+               autocorr, binfile, pimax, mu_max, nmu_bins, 0.0, numthreads};
+    const int st = run_core(c, o, &ex, R);
+    o->float_type = saved;
+    return st;
 }
 
 }  // namespace cfamd
@@ -2355,6 +2614,159 @@ int cf_amd_memcpy(void* dst, const void* src, size_t bytes, int kind)
             CF_CHECK(hipMemcpyAsync(dst, src, bytes, k, E.stream));
         }
         CF_CHECK(hipStreamSynchronize(E.stream));
+        return EXIT_SUCCESS;
+    });
+}
+
+void free_results_mocks(results_countpairs_mocks* r)
+{
+    if (!r) return;
+    free(r->npairs);
+    free(r->rupp);
+    free(r->rpavg);
+    free(r->weightavg);
+    r->npairs = nullptr;
+    r->rupp = r->rpavg = r->weightavg = nullptr;
+}
+
+void free_results_mocks_s_mu(results_countpairs_mocks_s_mu* r)
+{
+    if (!r) return;
+    free(r->npairs);
+    free(r->supp);
+    free(r->savg);
+    free(r->weightavg);
+    r->npairs = nullptr;
+    r->supp = r->savg = r->weightavg = nullptr;
+}
+
+int countpairs_mocks(const int64_t ND1, void* RA1, void* DEC1, void* CZ1, const int64_t ND2, void* RA2, void* DEC2,
+                     void* CZ2, const int numthreads, const int autocorr, const char* binfile, const double pimax,
+                     const int cosmology, results_countpairs_mocks* results, struct config_options* options,
+                     struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        const double t0 = now_s();
+        options->sort_on_z = 1;
+        fix_refine(options);
+        fix_maxcells(options);
+        if (options->fast_divide_and_NR_steps >= MAX_FAST_DIVIDE_NR_STEPS) options->fast_divide_and_NR_steps = 0;
+        CallOut R;
+        if (ND1 == 0 || (autocorr == 0 && ND2 == 0)) {
+            // nothing to count: bins and zero counts
+            double rmin, rmax;
+            int nbin;
+            if (setup_bins(binfile, &rmin, &rmax, &nbin, R.rupp) != EXIT_SUCCESS) return EXIT_FAILURE;
+            R.nbin = nbin;
+            R.n2 = (int)pimax;
+            const size_t tot = (size_t)(R.n2 + 1) * (nbin + 1);
+            R.np.assign(tot, 0);
+            R.avg.assign(tot, 0.0);
+            R.wavg.assign(tot, 0.0);
+            R.need_w = extra->weight_method != NONE;
+        } else {
+            if (sky_run(MODE_RPPI_MOCKS, ND1, RA1, DEC1, CZ1, ND2, RA2, DEC2, CZ2, numthreads, autocorr, binfile,
+                        pimax, 0.0, 0, cosmology, options, extra, R) != EXIT_SUCCESS) {
+                reset_bin_refine_factors(options);
+                return EXIT_FAILURE;
+            }
+            finalize(R, autocorr, ND1, options->need_avg_sep, R.n2 + 1);
+        }
+        const int npibin = R.n2, nbin = R.nbin;
+        const int64_t tot = (int64_t)(npibin + 1) * (nbin + 1);
+        results->nbin = nbin;
+        results->npibin = npibin;
+        results->pimax = pimax;
+        results->npairs = (uint64_t*)calloc(tot, sizeof(uint64_t));
+        results->rupp = (double*)malloc(sizeof(double) * nbin);
+        results->rpavg = (double*)calloc(tot, sizeof(double));
+        results->weightavg = (double*)calloc(tot, sizeof(double));
+        if (!results->npairs || !results->rupp || !results->rpavg || !results->weightavg) {
+            free_results_mocks(results);
+            return EXIT_FAILURE;
+        }
+        for (int i = 0; i < nbin; i++) {
+            results->rupp[i] = R.rupp[i];
+            for (int j = 0; j < npibin; j++) {
+                const int64_t idx = (int64_t)i * (npibin + 1) + j;
+                results->npairs[idx] = R.np[idx];
+                results->rpavg[idx] = options->need_avg_sep ? R.avg[idx] : 0.0;
+                results->weightavg[idx] = R.need_w ? R.wavg[idx] : 0.0;
+            }
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int countpairs_mocks_s_mu(const int64_t ND1, void* RA1, void* DEC1, void* CZ1, const int64_t ND2, void* RA2,
+                          void* DEC2, void* CZ2, const int numthreads, const int autocorr, const char* sbinfile,
+                          const double mu_max, const int nmu_bins, const int cosmology,
+                          results_countpairs_mocks_s_mu* results, struct config_options* options,
+                          struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        const double t0 = now_s();
+        options->sort_on_z = 1;
+        fix_refine(options);
+        fix_maxcells(options);
+        if (options->fast_divide_and_NR_steps >= MAX_FAST_DIVIDE_NR_STEPS) options->fast_divide_and_NR_steps = 0;
+        CallOut R;
+        if (ND1 == 0 || (autocorr == 0 && ND2 == 0)) {
+            double rmin, rmax;
+            int nbin;
+            if (setup_bins(sbinfile, &rmin, &rmax, &nbin, R.rupp) != EXIT_SUCCESS) return EXIT_FAILURE;
+            if (mu_max <= 0.0 || mu_max > 1.0 || nmu_bins < 1) {
+                fprintf(stderr, "Error: need 0 < mu_max <= 1 and nmu_bins >= 1 (got %lf, %d)\n", mu_max, nmu_bins);
+                return EXIT_FAILURE;
+            }
+            R.nbin = nbin;
+            R.n2 = nmu_bins;
+            const size_t tot = (size_t)(R.n2 + 1) * (nbin + 1);
+            R.np.assign(tot, 0);
+            R.avg.assign(tot, 0.0);
+            R.wavg.assign(tot, 0.0);
+            R.need_w = extra->weight_method != NONE;
+        } else {
+            if (sky_run(MODE_SMU_MOCKS, ND1, RA1, DEC1, CZ1, ND2, RA2, DEC2, CZ2, numthreads, autocorr, sbinfile, 0.0,
+                        mu_max, nmu_bins, cosmology, options, extra, R) != EXIT_SUCCESS) {
+                reset_bin_refine_factors(options);
+                return EXIT_FAILURE;
+            }
+            finalize(R, autocorr, ND1, options->need_avg_sep, R.n2 + 1);
+        }
+        const int n2 = R.n2, nbin = R.nbin;
+        const int64_t tot = (int64_t)(n2 + 1) * (nbin + 1);
+        results->nsbin = nbin;
+        results->nmu_bins = n2;
+        results->mu_max = mu_max;
+        results->mu_min = 0.0;
+        results->npairs = (uint64_t*)calloc(tot, sizeof(uint64_t));
+        results->supp = (double*)malloc(sizeof(double) * nbin);
+        results->savg = (double*)calloc(tot, sizeof(double));
+        results->weightavg = (double*)calloc(tot, sizeof(double));
+        if (!results->npairs || !results->supp || !results->savg || !results->weightavg) {
+            free_results_mocks_s_mu(results);
+            return EXIT_FAILURE;
+        }
+        for (int i = 0; i < nbin; i++) {
+            results->supp[i] = R.rupp[i];
+            for (int j = 0; j < n2; j++) {
+                const int64_t idx = (int64_t)i * (n2 + 1) + j;
+                results->npairs[idx] = R.np[idx];
+                results->savg[idx] = options->need_avg_sep ? R.avg[idx] : 0.0;
+                results->weightavg[idx] = R.need_w ? R.wavg[idx] : 0.0;
+            }
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
         return EXIT_SUCCESS;
     });
 }

@@ -446,6 +446,39 @@ __device__ __forceinline__ int find_kbin(const double* __restrict__ e, int nbin,
     return 0;
 }
 
+// On-sky pair (mocks/DDrppi_mocks/countpairs_rp_pi_mocks_kernels.c.src:1240-1290,
+// mocks/DDsmu_mocks/countpairs_s_mu_mocks_kernels.c.src fallback): s = x_j - x_i,
+// l = x_i + x_j; rp-pi: pi^2 = (s.l)^2 / |l|^2, rp^2 = s^2 - pi^2; s-mu:
+// mu^2 = (s.l)^2 / (|l|^2 s^2). Returns 0 for a rejected pair, else the squared
+// separation that selects the bin (*key: rp^2 or s^2) and the pi / mu column
+// (*j2; n2 where pi^2 >= pimax^2 or mu^2 >= mu_max^2, as the reference).
+__device__ __forceinline__ int los_split(const PairParams& P, int mode, double xi, double yi, double zi, double xj,
+                                         double yj, double zj, double* key, int* j2)
+{
+    const double parx = xi + xj, pary = yi + yj, parz = zi + zj;
+    const double perpx = xj - xi, perpy = yj - yi, perpz = zj - zi;
+    const double sqr_s = perpx * perpx + perpy * perpy + perpz * perpz;
+    const double s_dot_l = parx * perpx + pary * perpy + parz * perpz;
+    const double sqr_l = parx * parx + pary * pary + parz * parz;
+    const double sqr_s_dot_l = s_dot_l * s_dot_l;
+    const double sqr_lim = P.sqr_mu_max;
+    if (mode == MODE_RPPI_MOCKS) {
+        if (sqr_s >= P.sqr_rmax) return 0;
+        if (sqr_s_dot_l >= sqr_l * sqr_lim) return 0;
+        const double sqr_par = sqr_s_dot_l / sqr_l;
+        *j2 = sqr_par >= sqr_lim ? P.n2 : (int)(sqrt(sqr_par) * P.inv_d2);
+        const double sqr_perp = sqr_s - sqr_par;
+        if (sqr_perp >= P.sqr_rpmax || sqr_perp < P.sqr_rmin) return 0;
+        *key = sqr_perp;
+        return 1;
+    }
+    if (sqr_s >= P.sqr_rmax || sqr_s < P.sqr_rmin) return 0;
+    const double sqr_mu = sqr_s_dot_l / (sqr_l * sqr_s);
+    *j2 = sqr_mu >= sqr_lim ? P.n2 : (int)(sqrt(sqr_mu) * P.inv_d2);
+    *key = sqr_s;
+    return 1;
+}
+
 struct ITerms {
     double xpos, ypos, zpos;
     double max_dz, target;
@@ -514,6 +547,15 @@ __device__ __forceinline__ int replay_pair(const PairParams& P, int mode, const 
         if (t.skip) return -1;
         if (zj <= t.target) return -1;
         if (dz >= t.max_dz) return -1;
+    }
+    if (mode == MODE_RPPI_MOCKS || mode == MODE_SMU_MOCKS) {
+        double key;
+        int j2;
+        if (!los_split(P, mode, t.xpos, t.ypos, t.zpos, xj, yj, zj, &key, &j2)) return -1;
+        const int k = find_kbin(e, P.nbin, P.sorted_edges, key);
+        if (k == 0) return -1;
+        *sep = sqrt(key);
+        return k * (P.n2 + 1) + j2;
     }
     if (mode == MODE_SMU) {
         const double sqr_dz = dz * dz;
@@ -730,7 +772,28 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
                                            double yj, double zj, double wj, KernelLds& S)
 {
     constexpr bool is2d = MODE == MODE_RPPI || MODE == MODE_WP;
+    constexpr bool onsky = MODE == MODE_RPPI_MOCKS || MODE == MODE_SMU_MOCKS;
     const double dx = xj - xpos, dy = yj - ypos, dz = zj - zpos;
+    if (onsky) {
+        double key;
+        int j2;
+        if (!ok || !los_split(P, MODE, xpos, ypos, zpos, xj, yj, zj, &key, &j2)) return;
+        // pruning replay for pairs near the 3-D limit (the 3-D kernels' rules)
+        if (sep2<R2F>(dx, dy, dz) >= P.sqr_rmax - P.near_r2) {
+            const CellPairRec* c = fresh(cpp);
+            const DevLattice* l1 = fresh(l1p);
+            ITerms t = {};
+            i_terms(P, MODE, *c, l1->x[i], l1->y[i], l1->z[i], &t);
+            if (t.skip || zj <= t.target || dz >= t.max_dz) return;
+        }
+        const int k = lut_kbin(P, S, key);
+        if (k == 0) return;
+        const int a = (k * (P.n2 + 1) + j2) * (S.rmask + 1) + S.lane_copy;
+        atomicAdd(&S.np[a], 1ull);
+        if (P.need_avg) atomicAdd(&S.avg[a], fast_sqrt(key));
+        if (P.need_w) atomicAdd(&S.w[a], wi * wj);
+        return;
+    }
     const double sqr_dz = dz * dz;
     const double r2 = sep2<R2F>(dx, dy, dz);
     const double adz = fabs(dz);
@@ -1266,6 +1329,12 @@ template __global__ void k_pairs_tuned<MODE_SMU, R2_SUM_XY_Z, false>(PairParams,
 template __global__ void k_pairs_tuned<MODE_SMU, R2_FMA_DX_DY_DZ, false>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
     double*, unsigned long long*, int*, const int*);
+template __global__ void k_pairs_tuned<MODE_RPPI_MOCKS, R2_SUM_XY_Z, false>(PairParams, const DevLattice*,
+    const CellPairRec*, const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*,
+    double*, double*, unsigned long long*, int*, const int*);
+template __global__ void k_pairs_tuned<MODE_SMU_MOCKS, R2_SUM_XY_Z, false>(PairParams, const DevLattice*,
+    const CellPairRec*, const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*,
+    double*, double*, unsigned long long*, int*, const int*);
 template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true>(PairParams, const DevLattice*, const CellPairRec*,
     const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
     double*, unsigned long long*, int*, const int*);

@@ -53,12 +53,13 @@
 #endif
 
 #include "../include/corrfunc_theory.h"
+#include "../include/corrfunc_mocks.h"
 
 #define BOOST_CELL_THRESH 10
 #define BOOST_NUMPART_THRESH 250
 #define BOOST_BIN_REF 1
 
-enum { MODE_DD = 0, MODE_RPPI = 1, MODE_SMU = 2, MODE_XI = 3, MODE_WP = 4 };
+enum { MODE_DD = 0, MODE_RPPI = 1, MODE_SMU = 2, MODE_XI = 3, MODE_WP = 4, MODE_RPPI_MOCKS = 5, MODE_SMU_MOCKS = 6 };
 /* rounding-order ids */
 enum { R2_SUM_XY_Z = 0, R2_FMA_DZ_DY_DX = 1, R2_X_SUM_YZ = 2, R2_FMA_DX_DY_DZ = 3,
        RP_SUM = 10, RP_FMA = 11 };
@@ -354,6 +355,7 @@ typedef struct {
     int n2;             /* npibin or nmu_bins */
     double inv_d2;      /* 1/dpi or 1/dmu */
     double sqr_mu_max;
+    double sqr_rmax_bins;   /* on-sky rp-pi: rpmax^2 (sqr_rmax holds the 3-D limit) */
     int need_avg, need_w;
 } okparams;
 
@@ -500,6 +502,81 @@ static void kernel_cellpair(const okparams *P, const ocell *c0, const ocell *c1,
     }
 }
 
+/* On-sky pairs (mocks/DDrppi_mocks/countpairs_rp_pi_mocks_kernels.c.src:1124-1297,
+ * mocks/DDsmu_mocks/countpairs_s_mu_mocks_kernels.c.src fallback): the pruning of
+ * the 3-D kernel with the maximum 3-D separation (P->sqr_rmax holds
+ * rpmax^2 + pimax^2 for rp-pi, smax^2 for s-mu), then the line-of-sight split
+ * of s along l = x_i + x_j. P->pimax holds pimax (rp-pi) or mu_max (s-mu). */
+static void kernel_cellpair_mocks(const okparams *P, const ocell *c0, const ocell *c1, const ocellpair *cp, oacc *A)
+{
+    const int64_t N0 = c0->n, N1 = c1->n;
+    if (N0 == 0 || N1 == 0) return;
+    const double *x0 = c0->x, *y0 = c0->y, *z0 = c0->z, *w0 = c0->w;
+    const double *x1 = c1->x, *y1 = c1->y, *z1s = c1->z, *w1 = c1->w;
+    const double mxd = cp->min_dx, myd = cp->min_dy, mzd = cp->min_dz;
+    const int rppi = P->mode == MODE_RPPI_MOCKS;
+    const double sqr_max_sep = P->sqr_rmax;
+    const double sqr_lim = P->pimax * P->pimax;   /* pimax^2 or mu_max^2 */
+    const double max_all_dz = sqrt(sqr_max_sep - mxd * mxd - myd * myd);
+    int64_t z1 = 0;
+    for (int64_t i = 0; i < N0; i++) {
+        const double xpos = x0[i], ypos = y0[i], zpos = z0[i];
+        double max_dz = max_all_dz;
+        const double this_dz = z1s[z1] - zpos;
+        if (this_dz >= max_all_dz) continue;
+        if (cp->same == 1) {
+            z1++;
+        } else {
+            const double mdx = mxd > 0 ? mxd + fabs(xpos - cp->cx) : mxd;
+            const double mdy = myd > 0 ? myd + fabs(ypos - cp->cy) : myd;
+            const double mdz = mzd > 0 ? (this_dz > 0 ? this_dz : mzd + fabs(zpos - cp->cz)) : mzd;
+            if (mdx * mdx + mdy * mdy + mdz * mdz >= sqr_max_sep) continue;
+            max_dz = sqrt(sqr_max_sep - mdx * mdx - mdy * mdy);
+            const double target = zpos - max_all_dz;
+            while (z1 != N1 && z1s[z1] <= target) z1++;
+        }
+        if (z1 == N1) break;
+        int64_t j0 = z1;
+        const double target = zpos - max_dz;
+        while (j0 != N1 && z1s[j0] <= target) j0++;
+        const double wi = w0 ? w0[i] : 0.0;
+        for (int64_t j = j0; j < N1; j++) {
+            const double parx = xpos + x1[j], pary = ypos + y1[j], parz = zpos + z1s[j];
+            const double perpx = x1[j] - xpos, perpy = y1[j] - ypos, perpz = z1s[j] - zpos;
+            if (perpz >= max_dz) break;
+            const double sqr_s = perpx * perpx + perpy * perpy + perpz * perpz;
+            const double s_dot_l = parx * perpx + pary * perpy + parz * perpz;
+            const double sqr_l = parx * parx + pary * pary + parz * parz;
+            const double sqr_s_dot_l = s_dot_l * s_dot_l;
+            int k, ibin;
+            double sep;
+            if (rppi) {
+                if (sqr_s >= sqr_max_sep) continue;
+                if (sqr_s_dot_l >= sqr_l * sqr_lim) continue;
+                const double sqr_par = sqr_s_dot_l / sqr_l;
+                const int pb = sqr_par >= sqr_lim ? P->n2 : (int)(sqrt(sqr_par) * P->inv_d2);
+                const double sqr_perp = sqr_s - sqr_par;
+                if (sqr_perp >= P->sqr_rmax_bins || sqr_perp < P->sqr_rmin) continue;
+                k = find_kbin(P, sqr_perp);
+                if (k == 0) continue;
+                ibin = k * (P->n2 + 1) + pb;
+                sep = P->need_avg ? sqrt(sqr_perp) : 0.0;
+            } else {
+                if (sqr_s >= sqr_max_sep || sqr_s < P->sqr_rmin) continue;
+                const double sqr_mu = sqr_s_dot_l / (sqr_l * sqr_s);
+                const int mb = sqr_mu >= sqr_lim ? P->n2 : (int)(sqrt(sqr_mu) * P->inv_d2);
+                k = find_kbin(P, sqr_s);
+                if (k == 0) continue;
+                ibin = k * (P->n2 + 1) + mb;
+                sep = P->need_avg ? sqrt(sqr_s) : 0.0;
+            }
+            A->np[ibin]++;
+            if (P->need_avg) A->avg[ibin] += sep;
+            if (P->need_w) A->wavg[ibin] += wi * w1[j];
+        }
+    }
+}
+
 /* ------------------------------------------------------------ helpers */
 static int resolve_isa(int isa_in)
 {
@@ -572,7 +649,8 @@ typedef struct {
     int px, py, pz;
     double gx, gy, gz;            /* max sizes handed to gridlink */
     double max3, max2, max1;      /* cell-pair culling */
-    int boost_kind;               /* 0: DD-style (+1 on x,y), 1: DDsmu-style (x BOOST_BIN_REF) */
+    int boost_kind;               /* 0: DD-style (+1 on x,y), 1: DDsmu-style (x BOOST_BIN_REF),
+                                     2: on-sky style (+1 on x,y, density of the larger set) */
     int numthreads;
     okparams kp;
     int64_t totbins;
@@ -585,8 +663,11 @@ static int run_core(ocore *C, struct config_options *o, uint64_t *np, double *av
     if (gridlink(C->N1, C->X1, C->Y1, C->Z1, C->W1, C->xmin, C->xmax, C->ymin, C->ymax, C->zmin, C->zmax,
                  C->gx, C->gy, C->gz, C->xwrap, C->ywrap, C->zwrap, rx, ry, rz, o->max_cells_per_dim, &L1) != EXIT_SUCCESS)
         return EXIT_FAILURE;
-    if (C->boost_kind == 0) {
-        const double avg_np = ((double)C->N1) / (L1.nx * L1.ny * L1.nz);
+    if (C->boost_kind == 0 || C->boost_kind == 2) {
+        /* on-sky engines take the larger of the two sets' densities
+           (countpairs_rp_pi_mocks_impl.c.src:454-457) */
+        const double n_eff = C->boost_kind == 2 && C->N2 > C->N1 ? (double)C->N2 : (double)C->N1;
+        const double avg_np = n_eff / (L1.nx * L1.ny * L1.nz);
         const int max_nmesh = (int)fmax(L1.nx, fmax(L1.ny, L1.nz));
         if ((max_nmesh <= BOOST_CELL_THRESH || avg_np >= BOOST_NUMPART_THRESH) && max_nmesh < o->max_cells_per_dim) {
             if (get_bin_refine_scheme(o) == BINNING_DFL) {
@@ -661,7 +742,10 @@ static int run_core(ocore *C, struct config_options *o, uint64_t *np, double *av
 #pragma omp for schedule(dynamic)
 #endif
         for (int64_t k = 0; k < ncp; k++)
-            kernel_cellpair(&C->kp, &L1.cells[cps[k].c1], &Lb->cells[cps[k].c2], &cps[k], &A);
+            if (C->kp.mode >= MODE_RPPI_MOCKS)
+                kernel_cellpair_mocks(&C->kp, &L1.cells[cps[k].c1], &Lb->cells[cps[k].c2], &cps[k], &A);
+            else
+                kernel_cellpair(&C->kp, &L1.cells[cps[k].c1], &Lb->cells[cps[k].c2], &cps[k], &A);
     }
     for (int t = 0; t < nth; t++)
         for (int64_t b = 0; b < nb; b++) {
@@ -1184,3 +1268,336 @@ void oracle_free_results_rp_pi(results_countpairs_rp_pi *r) { free(r->npairs); f
 void oracle_free_results_s_mu(results_countpairs_s_mu *r) { free(r->npairs); free(r->supp); free(r->savg); free(r->weightavg); memset(r, 0, sizeof *r); }
 void oracle_free_results_xi(results_countpairs_xi *r) { free(r->npairs); free(r->xi); free(r->rupp); free(r->ravg); free(r->weightavg); memset(r, 0, sizeof *r); }
 void oracle_free_results_wp(results_countpairs_wp *r) { free(r->npairs); free(r->wp); free(r->rupp); free(r->rpavg); free(r->weightavg); memset(r, 0, sizeof *r); }
+
+/* ======================================================= on-sky (mocks) ==== */
+#define MOCKS_SPEED_OF_LIGHT 299800.0   /* utils/set_cosmo_dist.h:19 */
+#define MOCKS_PI_OVER_180 0.017453292519943295769236907684886127134428718885417254560971
+
+/* check_ra_dec_cz (countpairs_rp_pi_mocks_impl.c.src:43-101): fixes the caller's
+ * arrays in place, in their own precision. */
+static int mocks_check_ra_dec_cz(int64_t N, void *phi_v, void *theta_v, void *cz_v, size_t ft, int is_comoving)
+{
+    if (N == 0) return EXIT_SUCCESS;
+    if (phi_v == NULL || theta_v == NULL || cz_v == NULL) {
+        fprintf(stderr, "Input arrays can not be NULL.\n");
+        return EXIT_FAILURE;
+    }
+    int fix_cz = 0, fix_ra = 0, fix_dec = 0;
+    double max_cz = 0.0;
+    for (int64_t i = 0; i < N; i++) {
+        const double ph = ft == 4 ? ((float *)phi_v)[i] : ((double *)phi_v)[i];
+        const double th = ft == 4 ? ((float *)theta_v)[i] : ((double *)theta_v)[i];
+        const double c = ft == 4 ? ((float *)cz_v)[i] : ((double *)cz_v)[i];
+        if (c > max_cz) max_cz = c;
+        if (ph < 0.0) fix_ra = 1;
+        if (th > 90.0) fix_dec = 1;
+        if (th > 180) {
+            fprintf(stderr, "theta[%lld] = %g should be less than 180 deg\n", (long long)i, th);
+            return EXIT_FAILURE;
+        }
+    }
+    if (max_cz < 10.0 && is_comoving == 0) fix_cz = 1;
+    for (int64_t i = 0; i < N && (fix_cz || fix_ra || fix_dec); i++) {
+        if (ft == 4) {
+            float *ph = phi_v, *th = theta_v, *c = cz_v;
+            if (fix_ra) ph[i] += (float)180.0;
+            if (fix_dec) th[i] -= (float)90.0;
+            if (fix_cz) c[i] *= (float)MOCKS_SPEED_OF_LIGHT;
+        } else {
+            double *ph = phi_v, *th = theta_v, *c = cz_v;
+            if (fix_ra) ph[i] += 180.0;
+            if (fix_dec) th[i] -= 90.0;
+            if (fix_cz) c[i] *= MOCKS_SPEED_OF_LIGHT;
+        }
+    }
+    return EXIT_SUCCESS;
+}
+
+/* cosmology_params.c:26-56 */
+static int mocks_cosmology(int which, double *om, double *ol)
+{
+    switch (which) {
+    case 1: *om = 0.25; break;
+    case 2: *om = 0.302; break;
+    default:
+        fprintf(stderr, "ERROR: Cosmology=%d not implemented\n", which);
+        return EXIT_FAILURE;
+    }
+    *ol = 1.0 - *om;
+    return EXIT_SUCCESS;
+}
+
+/* set_cosmo_dist.c:25-71: comoving distance (Mpc/h) tabulated every 1/max_size in
+ * z by Simpson's rule on steps of 1e-2/max_size. Returns the table length. */
+static int mocks_cosmo_table(double zmax, int max_size, double *zc, double *dc, double om, double ol)
+{
+    int i = 0;
+    const double ok = 1.0 - om - ol;
+    const double Dh = MOCKS_SPEED_OF_LIGHT * 0.01 / 1.0;
+    const double Deltaz = 1.0 / max_size;
+    const double dz = 1e-2 * Deltaz;
+    double Eint = 0.0, E2 = 1.0, z2 = Deltaz;
+    for (double z = 2.0 * dz; z < zmax; z += 2.0 * dz) {
+        const double E0 = E2;
+        const double a = 1 + z - dz, b = 1 + z;
+        const double E1 = 1.0 / sqrt(om * (a * a * a) + ok * a + ol);
+        E2 = 1.0 / sqrt(om * (b * b * b) + ok * b + ol);
+        Eint += dz * (E0 + 4. * E1 + E2) / 3.;
+        if (z > (z2 - 1e-10) && z < (z2 + 1e-10)) {
+            if (i >= max_size) break;
+            zc[i] = z;
+            dc[i] = Eint * Dh;
+            z2 += Deltaz;
+            i++;
+        }
+    }
+    return i;
+}
+
+/* Linear interpolation in a table (the published textbook rule: bracket by
+ * bisection, y = y_lo + (x - x_lo) / (x_hi - x_lo) * (y_hi - y_lo)); x outside
+ * the table is an error. */
+static int mocks_interp(const double *xa, const double *ya, int n, double x, double *y)
+{
+    if (n < 2 || !(x >= xa[0] && x <= xa[n - 1])) return EXIT_FAILURE;
+    size_t lo = 0, hi = (size_t)n - 1;
+    while (hi > lo + 1) {
+        const size_t m = (hi + lo) / 2;
+        if (xa[m] > x) hi = m; else lo = m;
+    }
+    const double dx = xa[lo + 1] - xa[lo];
+    *y = dx > 0.0 ? ya[lo] + (x - xa[lo]) / dx * (ya[lo + 1] - ya[lo]) : 0.0;
+    return EXIT_SUCCESS;
+}
+
+/* (RA, DEC, cz or distance) -> Cartesian (countpairs_rp_pi_mocks_impl.c.src:309-390) */
+static int mocks_positions(int64_t N1, const double *ra1, const double *dec1, const double *cz1,
+                           int64_t N2, const double *ra2, const double *dec2, const double *cz2, int autocorr,
+                           int is_comoving, int cosmology, double **X1, double **Y1, double **Z1,
+                           double **X2, double **Y2, double **Z2)
+{
+    double om, ol;
+    if (mocks_cosmology(cosmology, &om, &ol) != EXIT_SUCCESS) return EXIT_FAILURE;
+    double *D1 = malloc(sizeof(double) * (size_t)(N1 > 0 ? N1 : 1));
+    double *D2 = autocorr ? D1 : malloc(sizeof(double) * (size_t)(N2 > 0 ? N2 : 1));
+    if (is_comoving) {
+        memcpy(D1, cz1, sizeof(double) * (size_t)N1);
+        if (!autocorr) memcpy(D2, cz2, sizeof(double) * (size_t)N2);
+    } else {
+        double czmax = 0.0;
+        const double inv_c = 1.0 / MOCKS_SPEED_OF_LIGHT;
+        for (int64_t i = 0; i < N1; i++) if (cz1[i] > czmax) czmax = cz1[i];
+        if (!autocorr) for (int64_t i = 0; i < N2; i++) if (cz2[i] > czmax) czmax = cz2[i];
+        const double zmax = czmax * inv_c + 0.01;
+        const int ws = 10000;
+        double *zc = calloc(ws, sizeof(double)), *dc = calloc(ws, sizeof(double));
+        const int nz = mocks_cosmo_table(zmax, ws, zc, dc, om, ol);
+        int bad = 0;
+        for (int64_t i = 0; i < N1 && !bad; i++) bad = mocks_interp(zc, dc, nz, cz1[i] * inv_c, &D1[i]);
+        if (!autocorr)
+            for (int64_t i = 0; i < N2 && !bad; i++) bad = mocks_interp(zc, dc, nz, cz2[i] * inv_c, &D2[i]);
+        free(zc); free(dc);
+        if (bad) {
+            fprintf(stderr, "oracle> cz outside the tabulated redshift range\n");
+            free(D1); if (!autocorr) free(D2);
+            return EXIT_FAILURE;
+        }
+    }
+    const int64_t ns[2] = {N1, autocorr ? 0 : N2};
+    const double *ra[2] = {ra1, ra2}, *dec[2] = {dec1, dec2}, *D[2] = {D1, D2};
+    double **out[2][3] = {{X1, Y1, Z1}, {X2, Y2, Z2}};
+    for (int s = 0; s < 2; s++) {
+        if (s == 1 && autocorr) { *X2 = *Y2 = *Z2 = NULL; break; }
+        const int64_t n = ns[s];
+        double *x = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        double *y = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        double *z = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        for (int64_t i = 0; i < n; i++) {
+            const double cd = cos(dec[s][i] * MOCKS_PI_OVER_180);
+            x[i] = D[s][i] * cd * cos(ra[s][i] * MOCKS_PI_OVER_180);
+            y[i] = D[s][i] * cd * sin(ra[s][i] * MOCKS_PI_OVER_180);
+            z[i] = D[s][i] * sin(dec[s][i] * MOCKS_PI_OVER_180);
+        }
+        *out[s][0] = x; *out[s][1] = y; *out[s][2] = z;
+    }
+    free(D1);
+    if (!autocorr) free(D2);
+    return EXIT_SUCCESS;
+}
+
+/* Shared driver of the two on-sky modes (countpairs_rp_pi_mocks_impl.c.src:200-847,
+ * countpairs_s_mu_mocks_impl.c.src:200-850). lim = pimax or mu_max; n2 = npibin
+ * or nmu_bins. Fills per-slot sums already doubled and averaged. */
+static int mocks_common(int mode, int64_t ND1, void *RA1, void *DEC1, void *CZ1, int64_t ND2, void *RA2,
+                        void *DEC2, void *CZ2, int numthreads, int autocorr, const char *binfile, double lim,
+                        int n2, int cosmology, struct config_options *options, struct extra_options *extra,
+                        double **rupp_out, int *nbin_out, uint64_t **np_out, double **avg_out, double **wavg_out)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    struct extra_options dummy;
+    if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+    const int need_w = extra->weight_method != NONE;
+    fix_options(options, 0);
+    const size_t ft = options->float_type;
+    if (mocks_check_ra_dec_cz(ND1, RA1, DEC1, CZ1, ft, options->is_comoving_dist) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (!autocorr && mocks_check_ra_dec_cz(ND2, RA2, DEC2, CZ2, ft, options->is_comoving_dist) != EXIT_SUCCESS)
+        return EXIT_FAILURE;
+    double om, ol;
+    if (mocks_cosmology(cosmology, &om, &ol) != EXIT_SUCCESS) return EXIT_FAILURE;
+    double *rupp = NULL, rmin = 0, rmax = 0;
+    int nbin = 0;
+    if (oracle_setup_bins(binfile, &rmin, &rmax, &nbin, &rupp) != EXIT_SUCCESS) return EXIT_FAILURE;
+    if (!(rmin > 0.0 && rmax > 0.0 && rmin < rmax && nbin > 0)) {
+        fprintf(stderr, "Error: Could not setup with bins correctly. (min = %lf, max = %lf, with nbins = %d).\n",
+                rmin, rmax, nbin);
+        free(rupp);
+        return EXIT_FAILURE;
+    }
+    if (mode == MODE_SMU_MOCKS) {
+        if (lim <= 0.0 || lim > 1.0) {
+            fprintf(stderr, "Error: max_mu must be greater than 0 and at most 1. The passed value is max_mu = %lf.\n", lim);
+            free(rupp);
+            return EXIT_FAILURE;
+        }
+        if (n2 < 1) {
+            fprintf(stderr, "Error: Number of mu bins = %d must be at least 1\n", n2);
+            free(rupp);
+            return EXIT_FAILURE;
+        }
+    }
+    const int64_t totbins = (int64_t)(n2 + 1) * (nbin + 1);
+    uint64_t *np = calloc((size_t)totbins, sizeof(uint64_t));
+    double *avg = calloc((size_t)totbins, sizeof(double)), *wavg = calloc((size_t)totbins, sizeof(double));
+    *rupp_out = rupp; *nbin_out = nbin; *np_out = np; *avg_out = avg; *wavg_out = wavg;
+    if (ND1 == 0 || (autocorr == 0 && ND2 == 0)) return EXIT_SUCCESS;
+
+    double *ra1 = upcast(RA1, ND1, ft), *dec1 = upcast(DEC1, ND1, ft), *cz1 = upcast(CZ1, ND1, ft);
+    double *w1 = need_w ? upcast(extra->weights0.weights[0], ND1, ft) : NULL;
+    double *ra2 = NULL, *dec2 = NULL, *cz2 = NULL, *w2 = NULL;
+    if (!autocorr) {
+        ra2 = upcast(RA2, ND2, ft); dec2 = upcast(DEC2, ND2, ft); cz2 = upcast(CZ2, ND2, ft);
+        w2 = need_w ? upcast(extra->weights1.weights[0], ND2, ft) : NULL;
+    }
+    double *X1 = NULL, *Y1 = NULL, *Z1 = NULL, *X2 = NULL, *Y2 = NULL, *Z2 = NULL;
+    double *rupp_sqr = malloc(sizeof(double) * nbin);
+    for (int i = 0; i < nbin; i++) rupp_sqr[i] = rupp[i] * rupp[i];
+    int status = EXIT_FAILURE;
+    if (mocks_positions(ND1, ra1, dec1, cz1, ND2, ra2, dec2, cz2, autocorr, options->is_comoving_dist, cosmology,
+                        &X1, &Y1, &Z1, &X2, &Y2, &Z2) != EXIT_SUCCESS)
+        goto out;
+    {
+        const double sqr_max_sep = mode == MODE_RPPI_MOCKS ? rmax * rmax + lim * lim : rmax * rmax;
+        const double max_sep = sqrt(sqr_max_sep);
+        ocore C;
+        memset(&C, 0, sizeof C);
+        C.mode = mode; C.autocorr = autocorr; C.N1 = ND1; C.N2 = ND2;
+        C.X1 = X1; C.Y1 = Y1; C.Z1 = Z1; C.W1 = w1;
+        C.X2 = autocorr ? NULL : X2; C.Y2 = autocorr ? NULL : Y2; C.Z2 = autocorr ? NULL : Z2; C.W2 = w2;
+        C.xmin = C.ymin = C.zmin = DBL_MAX; C.xmax = C.ymax = C.zmax = -DBL_MAX;
+        get_max_min(ND1, X1, Y1, Z1, &C.xmin, &C.ymin, &C.zmin, &C.xmax, &C.ymax, &C.zmax);
+        if (!autocorr) get_max_min(ND2, X2, Y2, Z2, &C.xmin, &C.ymin, &C.zmin, &C.xmax, &C.ymax, &C.zmax);
+        if (get_bin_refine_scheme(options) == BINNING_DFL) {
+            if (max_sep < 0.05 * (C.xmax - C.xmin)) options->bin_refine_factors[0] = 1;
+            if (max_sep < 0.05 * (C.ymax - C.ymin)) options->bin_refine_factors[1] = 1;
+            if (max_sep < 0.05 * (C.zmax - C.zmin)) options->bin_refine_factors[2] = 1;
+        }
+        C.gx = C.gy = C.gz = max_sep;
+        C.max3 = max_sep; C.max2 = -1.0; C.max1 = -1.0;
+        C.boost_kind = 2;
+        C.numthreads = numthreads;
+        C.kp.mode = mode; C.kp.r2form = R2_SUM_XY_Z; C.kp.float_index = 0;
+        C.kp.nbin = nbin; C.kp.rupp_sqr = rupp_sqr;
+        C.kp.sqr_rmax = sqr_max_sep; C.kp.sqr_rmax_bins = rmax * rmax; C.kp.sqr_rmin = rmin * rmin;
+        C.kp.pimax = lim; C.kp.n2 = n2;
+        {
+            const double d2 = mode == MODE_RPPI_MOCKS ? lim / n2 : lim / (double)n2;
+            C.kp.inv_d2 = 1.0 / d2;
+        }
+        C.kp.need_avg = options->need_avg_sep; C.kp.need_w = need_w;
+        C.totbins = totbins;
+        if (run_core(&C, options, np, avg, wavg) != EXIT_SUCCESS) goto out;
+        finalize_sums(autocorr, rupp[0], ND1, w1, options->need_avg_sep, need_w, n2 + 1, totbins, np, avg, wavg);
+    }
+    status = EXIT_SUCCESS;
+out:
+    free(ra1); free(dec1); free(cz1); free(w1); free(ra2); free(dec2); free(cz2); free(w2);
+    free(X1); free(Y1); free(Z1); free(X2); free(Y2); free(Z2); free(rupp_sqr);
+    reset_bin_refine_factors(options);
+    return status;
+}
+
+int oracle_countpairs_mocks(const int64_t ND1, void *RA1, void *DEC1, void *CZ1,
+                            const int64_t ND2, void *RA2, void *DEC2, void *CZ2,
+                            const int numthreads, const int autocorr, const char *binfile, const double pimax,
+                            const int cosmology, results_countpairs_mocks *results,
+                            struct config_options *options, struct extra_options *extra)
+{
+    const double t0 = now_s();
+    const int npibin = (int)pimax;
+    double *rupp = NULL, *avg = NULL, *wavg = NULL;
+    uint64_t *np = NULL;
+    int nbin = 0;
+    if (mocks_common(MODE_RPPI_MOCKS, ND1, RA1, DEC1, CZ1, ND2, RA2, DEC2, CZ2, numthreads, autocorr, binfile,
+                     pimax, npibin, cosmology, options, extra, &rupp, &nbin, &np, &avg, &wavg) != EXIT_SUCCESS) {
+        free(rupp); free(np); free(avg); free(wavg);
+        return EXIT_FAILURE;
+    }
+    const int64_t tot = (int64_t)(npibin + 1) * (nbin + 1);
+    const int need_w = extra != NULL && extra->weight_method != NONE;
+    results->nbin = nbin; results->npibin = npibin; results->pimax = pimax;
+    results->npairs = calloc((size_t)tot, sizeof(uint64_t));
+    results->rupp = malloc(sizeof(double) * nbin);
+    results->rpavg = calloc((size_t)tot, sizeof(double));
+    results->weightavg = calloc((size_t)tot, sizeof(double));
+    for (int i = 0; i < nbin; i++) {
+        results->rupp[i] = rupp[i];
+        for (int j = 0; j < npibin; j++) {
+            const int64_t k = (int64_t)i * (npibin + 1) + j;
+            results->npairs[k] = np[k];
+            results->rpavg[k] = options->need_avg_sep ? avg[k] : 0.0;
+            results->weightavg[k] = need_w ? wavg[k] : 0.0;
+        }
+    }
+    free(rupp); free(np); free(avg); free(wavg);
+    if (options->c_api_timer) options->c_api_time = now_s() - t0;
+    return EXIT_SUCCESS;
+}
+
+int oracle_countpairs_mocks_s_mu(const int64_t ND1, void *RA1, void *DEC1, void *CZ1,
+                                 const int64_t ND2, void *RA2, void *DEC2, void *CZ2,
+                                 const int numthreads, const int autocorr, const char *sbinfile, const double mu_max,
+                                 const int nmu_bins, const int cosmology, results_countpairs_mocks_s_mu *results,
+                                 struct config_options *options, struct extra_options *extra)
+{
+    const double t0 = now_s();
+    double *supp = NULL, *avg = NULL, *wavg = NULL;
+    uint64_t *np = NULL;
+    int nbin = 0;
+    if (mocks_common(MODE_SMU_MOCKS, ND1, RA1, DEC1, CZ1, ND2, RA2, DEC2, CZ2, numthreads, autocorr, sbinfile,
+                     mu_max, nmu_bins, cosmology, options, extra, &supp, &nbin, &np, &avg, &wavg) != EXIT_SUCCESS) {
+        free(supp); free(np); free(avg); free(wavg);
+        return EXIT_FAILURE;
+    }
+    const int64_t tot = (int64_t)(nmu_bins + 1) * (nbin + 1);
+    const int need_w = extra != NULL && extra->weight_method != NONE;
+    results->nsbin = nbin; results->nmu_bins = nmu_bins; results->mu_max = mu_max; results->mu_min = 0.0;
+    results->npairs = calloc((size_t)tot, sizeof(uint64_t));
+    results->supp = malloc(sizeof(double) * nbin);
+    results->savg = calloc((size_t)tot, sizeof(double));
+    results->weightavg = calloc((size_t)tot, sizeof(double));
+    for (int i = 0; i < nbin; i++) {
+        results->supp[i] = supp[i];
+        for (int j = 0; j < nmu_bins; j++) {
+            const int64_t k = (int64_t)i * (nmu_bins + 1) + j;
+            results->npairs[k] = np[k];
+            results->savg[k] = options->need_avg_sep ? avg[k] : 0.0;
+            results->weightavg[k] = need_w ? wavg[k] : 0.0;
+        }
+    }
+    free(supp); free(np); free(avg); free(wavg);
+    if (options->c_api_timer) options->c_api_time = now_s() - t0;
+    return EXIT_SUCCESS;
+}
+
+void oracle_free_results_mocks(results_countpairs_mocks *r) { free(r->npairs); free(r->rupp); free(r->rpavg); free(r->weightavg); memset(r, 0, sizeof *r); }
+void oracle_free_results_mocks_s_mu(results_countpairs_mocks_s_mu *r) { free(r->npairs); free(r->supp); free(r->savg); free(r->weightavg); memset(r, 0, sizeof *r); }

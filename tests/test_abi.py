@@ -15,7 +15,7 @@ INC = os.path.join(ROOT, "include")
 
 def declared_functions():
     names = []
-    for h in ("corrfunc_theory.h", "corrfunc_amd.h", "corrfunc_io.h"):
+    for h in ("corrfunc_theory.h", "corrfunc_amd.h", "corrfunc_io.h", "corrfunc_mocks.h"):
         text = open(os.path.join(INC, h)).read()
         text = re.sub(r"static inline[^{]*\{(?:[^{}]|\{[^{}]*\})*\}", "", text)
         names += re.findall(r"^\s*(?:int64_t|int|void|const char\s*\*)\s+\**\s*(\w+)\s*\(", text, re.M)
@@ -27,7 +27,8 @@ def test_library_exports_every_declared_symbol():
     names = declared_functions()
     assert {"countpairs", "countpairs_rp_pi", "countpairs_s_mu", "countpairs_xi", "countpairs_wp",
             "free_results", "free_results_rp_pi", "free_results_s_mu", "free_results_xi",
-            "free_results_wp", "cf_amd_last_stats", "read_positions", "read_columns_into_array"} <= set(names)
+            "free_results_wp", "cf_amd_last_stats", "read_positions", "read_columns_into_array", "countpairs_mocks", "countpairs_mocks_s_mu",
+            "free_results_mocks", "free_results_mocks_s_mu"} <= set(names)
     for n in names:
         assert hasattr(lib, n), n
     nm = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
