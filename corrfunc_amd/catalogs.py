@@ -100,3 +100,33 @@ def nfw_halos(n=16777216, L=1000.0, nhalo=20000, frac_halo=0.6, sigma_occ=1.0, r
     v /= np.linalg.norm(v, axis=1)[:, None]
     pos = np.concatenate([centres[k] + v * r[:, None], rng.uniform(0, L, (n - nh, 3))])
     return _wrap32(pos, L)
+
+
+def sky_mock(n=1000000, ra_range=(120.0, 240.0), dec_range=(0.0, 60.0), cz_range=(3000.0, 30000.0),
+             frac_clump=0.5, nclump=3000, sigma_deg=0.15, sigma_cz=200.0, seed=1006):
+    """float64 on-sky catalog (RA, DEC in degrees, cz in km/s) and pair_product
+    weights U(0.5, 1.5): (1 - frac_clump) of the points uniform in solid angle
+    and in volume (cz^3 uniform) over the window, the rest in `nclump` Gaussian
+    clumps (sigma_deg on the sky, sigma_cz along the line of sight), clipped to
+    the window."""
+    rng = np.random.default_rng(seed)
+    s0, s1 = np.sin(np.radians(dec_range[0])), np.sin(np.radians(dec_range[1]))
+
+    def uniform_sky(m):
+        ra = rng.uniform(ra_range[0], ra_range[1], m)
+        dec = np.degrees(np.arcsin(rng.uniform(s0, s1, m)))
+        cz = np.cbrt(rng.uniform(cz_range[0] ** 3, cz_range[1] ** 3, m))
+        return ra, dec, cz
+
+    nc = int(round(frac_clump * n))
+    cra, cdec, ccz = uniform_sky(nclump)
+    k = rng.integers(0, nclump, nc)
+    ra_c = np.clip(cra[k] + rng.normal(0, sigma_deg, nc), *ra_range)
+    dec_c = np.clip(cdec[k] + rng.normal(0, sigma_deg, nc), *dec_range)
+    cz_c = np.clip(ccz[k] + rng.normal(0, sigma_cz, nc), *cz_range)
+    ra_u, dec_u, cz_u = uniform_sky(n - nc)
+    ra = np.concatenate([ra_c, ra_u])
+    dec = np.concatenate([dec_c, dec_u])
+    cz = np.concatenate([cz_c, cz_u])
+    w = rng.uniform(0.5, 1.5, n)
+    return [ra, dec, cz], w

@@ -1,9 +1,11 @@
-// Command-line programs DD, DDrppi, DDsmu, wp and xi on top of the engine
-// library: same arguments, defaults, checks and output columns as the
-// reference's theory/DD/DD.c, DDrppi/DDrppi.c, DDsmu/DDsmu.c, wp/wp.c and
-// xi/xi.c (built with theory.options: periodic, positions read as float unless
-// DOUBLE_PREC, averages printed only with OUTPUT_RPAVG). One source, built once
-// per program with CF_CLI set to the program's number.
+// Command-line programs DD, DDrppi, DDsmu, wp, xi, DDrppi_mocks and DDsmu_mocks
+// on top of the engine library: same arguments, defaults, checks and output
+// columns as the reference's theory/DD/DD.c, DDrppi/DDrppi.c, DDsmu/DDsmu.c,
+// wp/wp.c, xi/xi.c (built with theory.options: periodic, positions read as float
+// unless DOUBLE_PREC, averages printed only with OUTPUT_RPAVG) and
+// mocks/DDrppi_mocks/DDrppi_mocks.c, mocks/DDsmu_mocks/DDsmu_mocks.c (built with
+// mocks.options: DOUBLE_PREC and OUTPUT_RPAVG). One source, built once per
+// program with CF_CLI set to the program's number.
 #include <cassert>
 #include <cinttypes>
 #include <cmath>
@@ -13,6 +15,7 @@
 #include <sys/time.h>
 
 #include "../../include/corrfunc_io.h"
+#include "../../include/corrfunc_mocks.h"
 #include "../../include/corrfunc_theory.h"
 
 #define CLI_DD 1
@@ -20,8 +23,10 @@
 #define CLI_DDSMU 3
 #define CLI_WP 4
 #define CLI_XI 5
+#define CLI_DDRPPI_MOCKS 6
+#define CLI_DDSMU_MOCKS 7
 #ifndef CF_CLI
-#error "CF_CLI selects the program (1 DD, 2 DDrppi, 3 DDsmu, 4 wp, 5 xi)"
+#error "CF_CLI selects the program (1 DD, 2 DDrppi, 3 DDsmu, 4 wp, 5 xi, 6 DDrppi_mocks, 7 DDsmu_mocks)"
 #endif
 
 #ifdef DOUBLE_PREC
@@ -39,7 +44,8 @@ double seconds_since(const timeval& t0)
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-6 * (double)(t1.tv_usec - t0.tv_usec);
 }
 
-#if CF_CLI == CLI_DD || CF_CLI == CLI_DDRPPI || CF_CLI == CLI_DDSMU
+#if CF_CLI == CLI_DD || CF_CLI == CLI_DDRPPI || CF_CLI == CLI_DDSMU || CF_CLI == CLI_DDRPPI_MOCKS || \
+    CF_CLI == CLI_DDSMU_MOCKS
 constexpr bool kTwoSets = true;
 #else
 constexpr bool kTwoSets = false;
@@ -70,6 +76,19 @@ const Program kProg = {"DDsmu",
                        "[weight_method weights_file1 weights_format1 [weights_file2 weights_format2]] > DDfile",
                        {"file1", "format1", "file2", "format2", "sbinfile", "mu_max", "nmu_bins", "boxsize",
                         "Nthreads"},
+                       9};
+#elif CF_CLI == CLI_DDRPPI_MOCKS
+const Program kProg = {"DDrppi_mocks",
+                       "DDrppi_mocks file1 format1 file2 format2 binfile pimax cosmology numthreads [weight_method "
+                       "weights_file1 weights_format1 [weights_file2 weights_format2]] > DDfile",
+                       {"file1", "format1", "file2", "format2", "binfile", "pimax", "cosmology flag", "numthreads"},
+                       8};
+#elif CF_CLI == CLI_DDSMU_MOCKS
+const Program kProg = {"DDsmu_mocks",
+                       "DDsmu_mocks file1 format1 file2 format2 sbinfile mu_max nmu_bins cosmology numthreads "
+                       "[weight_method weights_file1 weights_format1 [weights_file2 weights_format2]] > DDfile",
+                       {"file1", "format1", "file2", "format2", "sbinfile", "mu_max", "nmu_bins", "cosmology flag",
+                        "numthreads"},
                        9};
 #elif CF_CLI == CLI_WP
 const Program kProg = {"wp",
@@ -157,9 +176,11 @@ int main(int argc, char* argv[])
     DOUBLE pimax = 40.0;
     double mu_max = 1.0;
     int nmu_bins = 1;
+    int cosmology = 1;
     (void)pimax;
     (void)mu_max;
     (void)nmu_bins;
+    (void)cosmology;
 #if CF_CLI == CLI_WP || CF_CLI == CLI_XI
     boxsize = atof(argv[1]);
     file1 = file2 = argv[2];
@@ -177,6 +198,15 @@ int main(int argc, char* argv[])
     binfile = argv[5];
 #if CF_CLI == CLI_DD
     boxsize = atof(argv[6]);
+#elif CF_CLI == CLI_DDRPPI_MOCKS
+    sscanf(argv[6], sizeof(DOUBLE) == 8 ? "%lf" : "%f", &pimax);
+    cosmology = atoi(argv[7]);
+    boxsize = 0.0;
+#elif CF_CLI == CLI_DDSMU_MOCKS
+    sscanf(argv[6], "%lf", &mu_max);
+    sscanf(argv[7], "%d", &nmu_bins);
+    cosmology = atoi(argv[8]);
+    boxsize = 0.0;
 #elif CF_CLI == CLI_DDRPPI
     sscanf(argv[6], sizeof(DOUBLE) == 8 ? "%lf" : "%f", &pimax);
     boxsize = atof(argv[7]);
@@ -270,6 +300,14 @@ int main(int argc, char* argv[])
     results_countpairs_s_mu res;
     status = countpairs_s_mu(ND1, x1, y1, z1, ND2, x2, y2, z2, nthreads, autocorr, binfile, mu_max, nmu_bins, &res,
                              &options, &extra);
+#elif CF_CLI == CLI_DDRPPI_MOCKS
+    results_countpairs_mocks res;
+    status = countpairs_mocks(ND1, x1, y1, z1, ND2, x2, y2, z2, nthreads, autocorr, binfile, pimax, cosmology, &res,
+                              &options, &extra);
+#elif CF_CLI == CLI_DDSMU_MOCKS
+    results_countpairs_mocks_s_mu res;
+    status = countpairs_mocks_s_mu(ND1, x1, y1, z1, ND2, x2, y2, z2, nthreads, autocorr, binfile, mu_max, nmu_bins,
+                                   cosmology, &res, &options, &extra);
 #elif CF_CLI == CLI_WP
     results_countpairs_wp res;
     status = countpairs_wp(ND1, x1, y1, z1, boxsize, nthreads, binfile, pimax, &res, &options, &extra);
@@ -323,6 +361,28 @@ int main(int argc, char* argv[])
         smin = smax;
     }
     free_results_s_mu(&res);
+#elif CF_CLI == CLI_DDRPPI_MOCKS
+    const double dpi = pimax / (double)res.npibin;
+    for (int i = 1; i < res.nbin; i++) {
+        const double logrp = log10(res.rupp[i]);
+        for (int j = 0; j < res.npibin; j++) {
+            const int k = i * (res.npibin + 1) + j;
+            fprintf(stdout, "%10" PRIu64 " %20.8lf %20.8lf  %20.8lf %20.8lf \n", res.npairs[k], res.rpavg[k], logrp,
+                    (j + 1) * dpi, res.weightavg[k]);
+        }
+    }
+    free_results_mocks(&res);
+#elif CF_CLI == CLI_DDSMU_MOCKS
+    const double dmu = mu_max / (double)res.nmu_bins;
+    for (int i = 1; i < res.nsbin; i++) {
+        const double log_supp = log10(res.supp[i]);
+        for (int j = 0; j < res.nmu_bins; j++) {
+            const int k = i * (res.nmu_bins + 1) + j;
+            fprintf(stdout, "%10" PRIu64 " %20.8lf %20.8lf  %20.8lf %20.8lf \n", res.npairs[k], res.savg[k], log_supp,
+                    (j + 1) * dmu, res.weightavg[k]);
+        }
+    }
+    free_results_mocks_s_mu(&res);
 #elif CF_CLI == CLI_WP
     double rlow = res.rupp[0];
     for (int i = 1; i < res.nbin; ++i) {

@@ -17,7 +17,9 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1903,7 +1905,7 @@ struct SkySet {
 // Both sets' Cartesian positions (and FP64 weights).
 static int sky_positions(int64_t n1, const void* ra1, const void* dec1, const void* cz1, const void* w1, int64_t n2,
                          const void* ra2, const void* dec2, const void* cz2, const void* w2, int autocorr,
-                         size_t esize, int is_comoving, int cosmology, SkySet& s1, SkySet& s2)
+                         size_t esize, int is_comoving, int cosmology, int nthreads, SkySet& s1, SkySet& s2)
 {
     double om, ol;
     if (sky_cosmology(cosmology, &om, &ol) != EXIT_SUCCESS) return EXIT_FAILURE;
@@ -1925,19 +1927,32 @@ static int sky_positions(int64_t n1, const void* ra1, const void* dec1, const vo
         S.x.resize(n);
         S.y.resize(n);
         S.z.resize(n);
-        for (int64_t i = 0; i < n; i++) {
-            double d = sky_value(cz, esize, i);
-            if (!is_comoving && !sky_interp(zc, dc, d * inv_c, &d)) {
-                fprintf(stderr, "Error: cz = %g is outside the tabulated redshift range [%g, %g] of the distance "
-                                "conversion\n", sky_value(cz, esize, i), zc.empty() ? 0.0 : zc.front() * kSpeedOfLight,
-                        zc.empty() ? 0.0 : zc.back() * kSpeedOfLight);
-                return EXIT_FAILURE;
+        // element-wise, so the result does not depend on the split across host threads
+        std::atomic<int64_t> bad{-1};
+        auto convert = [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; i++) {
+                double d = sky_value(cz, esize, i);
+                if (!is_comoving && !sky_interp(zc, dc, d * inv_c, &d)) {
+                    bad = i;
+                    return;
+                }
+                const double de = sky_value(dec, esize, i), ra_i = sky_value(ra, esize, i);
+                const double cd = cos(de * kPiOver180);
+                S.x[i] = d * cd * cos(ra_i * kPiOver180);
+                S.y[i] = d * cd * sin(ra_i * kPiOver180);
+                S.z[i] = d * sin(de * kPiOver180);
             }
-            const double de = sky_value(dec, esize, i), ra_i = sky_value(ra, esize, i);
-            const double cd = cos(de * kPiOver180);
-            S.x[i] = d * cd * cos(ra_i * kPiOver180);
-            S.y[i] = d * cd * sin(ra_i * kPiOver180);
-            S.z[i] = d * sin(de * kPiOver180);
+        };
+        const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, n / 65536));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(convert, n * t / nt, n * (t + 1) / nt);
+        convert(0, n / nt);
+        for (auto& th : pool) th.join();
+        if (bad >= 0) {
+            fprintf(stderr, "Error: cz = %g is outside the tabulated redshift range [%g, %g] of the distance "
+                            "conversion\n", sky_value(cz, esize, bad), zc.empty() ? 0.0 : zc.front() * kSpeedOfLight,
+                    zc.empty() ? 0.0 : zc.back() * kSpeedOfLight);
+            return EXIT_FAILURE;
         }
         if (w) {
             S.w.resize(n);
@@ -1966,7 +1981,7 @@ static int sky_run(int mode, int64_t ND1, void* RA1, void* DEC1, void* CZ1, int6
     SkySet s1, s2;
     if (sky_positions(ND1, RA1, DEC1, CZ1, need_w ? extra->weights0.weights[0] : nullptr, ND2, RA2, DEC2, CZ2,
                       need_w && !autocorr ? extra->weights1.weights[0] : nullptr, autocorr, esize,
-                      o->is_comoving_dist, cosmology, s1, s2) != EXIT_SUCCESS)
+                      o->is_comoving_dist, cosmology, numthreads, s1, s2) != EXIT_SUCCESS)
         return EXIT_FAILURE;
     if (need_w && (s1.w.size() != (size_t)ND1 || (!autocorr && s2.w.size() != (size_t)ND2))) {
         fprintf(stderr, "Error: weight_method requires weights arrays\n");

@@ -60,3 +60,22 @@ def convert_rp_pi_counts_to_wp(ND1, ND2, NR1, NR2, D1D2, D1R2, D2R1, R1R2, nrpbi
         raise ValueError("Pimax = {0} should be equal to the product of npibins = {1} and dpi = {2}. Check your "
                          "binning scheme.".format(pimax, npibins, dpi))
     return 2.0 * dpi * np.array([np.sum(xirppi[i * npibins:(i + 1) * npibins]) for i in range(nrpbins)])
+
+
+def fix_ra_dec(ra, dec):
+    """Wraps RA into [0, 360] and DEC into [-90, 90] in place, as the reference's
+    Corrfunc/utils.py:421-461 does: negative RA are shifted by +180 and DEC above
+    90 by -90 (every element, once). Returns the arrays cast to RA's dtype."""
+    try:
+        input_dtype = ra.dtype
+    except AttributeError:
+        raise TypeError("Input RA array must be a numpy array")
+    if ra is None or dec is None:
+        raise ValueError("RA or DEC must be valid arrays")
+    if np.size(ra) and np.min(ra) < 0.0:
+        print("Warning: found negative RA values, wrapping into [0.0, 360.0]  range")
+        ra += 180.0
+    if np.size(dec) and np.max(dec) > 90.0:
+        print("Warning: found DEC values more than 90.0; wrapping into [-90.0, 90.0] range")
+        dec -= 90.0
+    return ra.astype(input_dtype), dec.astype(input_dtype)

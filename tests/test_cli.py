@@ -1,4 +1,5 @@
-"""Command-line programs DD, DDrppi, DDsmu, wp and xi (corrfunc_amd/bin).
+"""Command-line programs DD, DDrppi, DDsmu, wp, xi, DDrppi_mocks and DDsmu_mocks
+(corrfunc_amd/bin).
 
 CPU tests: argument checking and read errors, which end before any pair
 counting (the reference programs' rules, theory/DD/DD.c:85-131 and siblings).
@@ -17,8 +18,8 @@ from helpers import weights, write_fastfood
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "corrfunc_amd", "bin")
-PROGS = ("DD", "DDrppi", "DDsmu", "wp", "xi")
-NARGS = {"DD": 7, "DDrppi": 8, "DDsmu": 9, "wp": 6, "xi": 5}
+PROGS = ("DD", "DDrppi", "DDsmu", "wp", "xi", "DDrppi_mocks", "DDsmu_mocks")
+NARGS = {"DD": 7, "DDrppi": 8, "DDsmu": 9, "wp": 6, "xi": 5, "DDrppi_mocks": 8, "DDsmu_mocks": 9}
 
 
 def prog(name):
@@ -56,6 +57,10 @@ def _args(name, f1, f2, fmt, binfile, L, nthreads=2):
         return [f1, fmt, f2, fmt, binfile, 1.0, 5, L, nthreads]
     if name == "wp":
         return [L, f1, fmt, binfile, 20.0, nthreads]
+    if name == "DDrppi_mocks":
+        return [f1, fmt, f2, fmt, binfile, 20.0, 1, nthreads]
+    if name == "DDsmu_mocks":
+        return [f1, fmt, f2, fmt, binfile, 0.9, 6, 2, nthreads]
     return [L, f1, fmt, binfile, nthreads]
 
 
@@ -87,7 +92,7 @@ MODE = {"DD": "countpairs", "DDrppi": "countpairs_rp_pi", "DDsmu": "countpairs_s
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("cross", [False, True])
-@pytest.mark.parametrize("name", PROGS)
+@pytest.mark.parametrize("name", PROGS[:5])
 def test_program_matches_oracle(name, cross, tmp_path, oracle):
     if cross and name in ("wp", "xi"):
         pytest.skip("wp and xi are autocorrelations")
@@ -145,3 +150,55 @@ def test_program_matches_oracle(name, cross, tmp_path, oracle):
         assert _close(got[:, 1], np.tile(np.arange(1, 21, dtype=np.float64), len(edges) - 1))
     if name == "DDsmu":
         assert _close(got[:, 2], np.tile(np.arange(1, 6) / 5.0, len(edges) - 1))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cross", [False, True])
+@pytest.mark.parametrize("name", ["DDrppi_mocks", "DDsmu_mocks"])
+def test_mocks_program_matches_oracle(name, cross, tmp_path, oracle):
+    """On-sky programs (double positions, averages printed): fast-food RA / DEC /
+    cz catalogs, ascii weights; columns npairs, average, log10(upper edge),
+    pi or mu upper edge, weightavg."""
+    rng = np.random.default_rng(91 + len(name))
+
+    def cat(n):
+        return [rng.uniform(150, 200, n), rng.uniform(10, 40, n), rng.uniform(6000, 12000, n)]
+    A, B = cat(5000), cat(4000)
+    W1, W2 = rng.uniform(0.5, 1.5, 5000), rng.uniform(0.5, 1.5, 4000)
+    f1, f2 = str(tmp_path / "s1.ff"), str(tmp_path / "s2.ff")
+    write_fastfood(f1, A)
+    write_fastfood(f2, B)
+    w1, w2 = str(tmp_path / "w1.txt"), str(tmp_path / "w2.txt")
+    np.savetxt(w1, W1, fmt="%.17g")
+    np.savetxt(w2, W2, fmt="%.17g")
+    edges = np.logspace(np.log10(0.5), np.log10(15.0), 7)
+    binfile = str(tmp_path / "bins")
+    with open(binfile, "w") as f:
+        for lo, hi in zip(edges[:-1], edges[1:]):
+            f.write("%r %r\n" % (float(lo), float(hi)))
+    args = _args(name, f1, f2 if cross else f1, "f", binfile, 0.0)
+    args += ["pair_product", w1, "a"] + ([w2, "a"] if cross else [])
+    r = run(name, *args)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = _parse(r.stdout)
+    mode = "countpairs_mocks" if name == "DDrppi_mocks" else "countpairs_mocks_s_mu"
+    kw = dict(autocorr=0 if cross else 1, nthreads=2, binfile=binfile, RA1=A[0], DEC1=A[1], CZ1=A[2], weights1=W1,
+              weight_type="pair_product")
+    if cross:
+        kw.update(RA2=B[0], DEC2=B[1], CZ2=B[2], weights2=W2)
+    if mode == "countpairs_mocks":
+        kw.update(pimax=20.0, cosmology=1)
+    else:
+        kw.update(mu_max=0.9, nmu_bins=6, cosmology=2)
+    o = _abi.make_options(periodic=False, need_avg_sep=True, isa="fallback")
+    want = _abi.call_mocks(oracle, mode, "oracle_", options=o, **kw)
+    s = _abi.reported_slots(mode, want)
+    assert got.shape[0] == len(s)
+    assert np.array_equal(got[:, 0].astype(np.uint64), want["npairs"][s])
+    assert want["npairs"][s].sum() > 0
+    assert np.all(np.abs(got[:, 1] - want["ravg"][s]) <= 1e-8 + 1e-9 * np.abs(want["ravg"][s]))
+    assert np.all(np.abs(got[:, 4] - want["weightavg"][s]) <= 1e-8 + 1e-9 * np.abs(want["weightavg"][s]))
+    m = 20 if mode == "countpairs_mocks" else 6
+    assert np.allclose(got[:, 2], np.repeat(np.log10(edges[1:]), m), atol=1e-8)
+    step = 1.0 if mode == "countpairs_mocks" else 0.15
+    assert np.allclose(got[:, 3], np.tile(np.arange(1, m + 1) * step, len(edges) - 1), atol=1e-8)

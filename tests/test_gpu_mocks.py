@@ -175,3 +175,41 @@ def test_errors(engine, tmp_path, capfd):
     assert _abi.call_mocks(engine, "countpairs_mocks_s_mu", binfile=good, mu_max=1.5, nmu_bins=5, cosmology=1,
                            options=opts(False), **kw) is None
     capfd.readouterr()
+
+
+def test_python_wrappers(tmp_path):
+    """corrfunc_amd.mocks.DDrppi_mocks / DDsmu_mocks: the doctest call of the
+    reference wrapper (Corrfunc/mocks/DDrppi_mocks.py:241-317), and DDsmu_mocks
+    against the C entry point."""
+    from corrfunc_amd.mocks import DDrppi_mocks, DDsmu_mocks
+    _select("tuned")
+    N, boxsize = 100000, 420.0
+    np.random.seed(42)
+    X = np.random.uniform(-0.5 * boxsize, 0.5 * boxsize, N)
+    Y = np.random.uniform(-0.5 * boxsize, 0.5 * boxsize, N)
+    Z = np.random.uniform(-0.5 * boxsize, 0.5 * boxsize, N)
+    weights = np.ones_like(X)
+    CZ = np.sqrt(X * X + Y * Y + Z * Z)
+    inv_cz = 1.0 / CZ
+    X *= inv_cz
+    Y *= inv_cz
+    Z *= inv_cz
+    DEC = 90.0 - np.arccos(Z) * 180.0 / math.pi
+    RA = (np.arctan2(Y, X) * 180.0 / math.pi) + 180.0
+    binfile = bins_file(tmp_path, F["bins"])
+    results = DDrppi_mocks(1, 1, 2, 40.0, binfile, RA, DEC, CZ, weights1=weights, weight_type='pair_product',
+                           output_rpavg=True, is_comoving_dist=True)
+    assert results.dtype.names == ("rmin", "rmax", "rpavg", "pimax", "npairs", "weightavg")
+    rows = np.array(F["ddrppi_doctest"]["rows"])
+    got = results[519:]
+    assert np.array_equal(got["npairs"], rows[:, 4].astype(np.uint64))
+    for k, name in enumerate(("rmin", "rmax", "rpavg", "pimax")):
+        assert np.array_equal(np.round(got[name], 6 if k < 3 else 1), rows[:, k]), name
+    res2, t = DDsmu_mocks(1, 2, 2, 0.5, 5, binfile, RA, DEC, CZ, is_comoving_dist=True, output_savg=True,
+                          c_api_timer=True)
+    assert res2.dtype.names == ("smin", "smax", "savg", "mumax", "npairs", "weightavg") and t > 0
+    raw = _abi.call_mocks(_lib.lib(), "countpairs_mocks_s_mu", autocorr=1, nthreads=2, binfile=binfile, RA1=RA,
+                          DEC1=DEC, CZ1=CZ, mu_max=0.5, nmu_bins=5, cosmology=2, options=opts(True))
+    s = _abi.reported_slots("countpairs_mocks_s_mu", raw)
+    assert np.array_equal(res2["npairs"], raw["npairs"][s])
+    assert np.allclose(res2["mumax"][:5], [0.1, 0.2, 0.3, 0.4, 0.5])
