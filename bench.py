@@ -463,8 +463,9 @@ def main():
                          "(the known-size kernel that scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; the multi-GPU design is described in DESIGN.md "
-              "(no figure at %d GPUs is produced)" % args.gpus, file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; several GPUs would split the cell-pair work list and "
+              "sum the histograms with one all-reduce (no figure at %d GPUs is produced)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     from corrfunc_amd import _lib, scheduler

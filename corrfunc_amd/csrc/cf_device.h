@@ -9,7 +9,7 @@ namespace cfamd {
 // RA, DEC, distance), (rp, pi) / (s, mu) split along the pair's mean line of sight.
 enum Mode { MODE_DD = 0, MODE_RPPI = 1, MODE_SMU = 2, MODE_XI = 3, MODE_WP = 4, MODE_RPPI_MOCKS = 5, MODE_SMU_MOCKS = 6 };
 
-// FP64 rounding orders of the reference per-ISA kernels.
+// FP64 rounding orders of the reference per-ISA kernels (which mode and ISA uses which: r2form_for).
 enum R2Form {
     R2_SUM_XY_Z = 0,      // (dx*dx + dy*dy) + dz*dz          fallback / SSE / AVX (DD, DDsmu)
     R2_FMA_DZ_DY_DX = 1,  // fma(dz,dz,fma(dy,dy,dx*dx))      AVX512F DD, xi
