@@ -8,7 +8,8 @@ Same arguments, checks, NaN convention and errors as the reference.
 """
 import numpy as np
 
-__all__ = ("convert_3d_counts_to_cf", "convert_rp_pi_counts_to_wp")
+__all__ = ("convert_3d_counts_to_cf", "convert_rp_pi_counts_to_wp", "translate_isa_string_to_enum",
+           "return_file_with_rbins", "fix_ra_dec", "fix_cz", "convert_to_native_endian", "is_native_endian")
 
 
 def _npairs(a):
@@ -79,3 +80,73 @@ def fix_ra_dec(ra, dec):
         print("Warning: found DEC values more than 90.0; wrapping into [-90.0, 90.0] range")
         dec -= 90.0
     return ra.astype(input_dtype), dec.astype(input_dtype)
+
+
+def fix_cz(cz):
+    """Multiplies cz by the speed of light (299800 km/s) in place when its maximum
+    is below 10, i.e. when redshifts were passed (Corrfunc/utils.py:384-418).
+    Returns the array cast to its own dtype."""
+    try:
+        input_dtype = cz.dtype
+    except AttributeError:
+        raise TypeError("Input cz array must be a numpy array")
+    if np.max(cz) < 10.0:
+        cz *= 299800.0
+    return cz.astype(input_dtype)
+
+
+def translate_isa_string_to_enum(isa):
+    """Instruction-set name -> the C API's enum value (Corrfunc/utils.py:464-518):
+    FASTEST -1, FALLBACK 0, SSE42 6, AVX 7, AVX2 8, AVX512F 9. The engine uses the
+    value to pick which FP64 rounding order of the reference it reproduces."""
+    from .theory import _common
+    return _common.translate_isa_string_to_enum(isa)
+
+
+def return_file_with_rbins(rbins):
+    """(binfile, delete_after_use) as Corrfunc/utils.py:324-381: an existing file
+    name is returned as is (IOError if it does not exist); an array of >= 1 edges
+    is sorted and written as consecutive "lo hi" lines to a temporary file that
+    the caller deletes."""
+    import os
+    import tempfile
+    if isinstance(rbins, str):
+        if os.path.exists(rbins):
+            return rbins, False
+        raise IOError("Could not find file = `{0}` containing the bins".format(rbins))
+    if len(rbins) >= 1:
+        rbins = sorted(rbins)
+        with tempfile.NamedTemporaryFile(delete=False, mode="w") as f:
+            for i in range(len(rbins) - 1):
+                f.write("{0} {1}\n".format(rbins[i], rbins[i + 1]))
+            name = f.name
+        return name, True
+    raise TypeError("Input `binfile` was not a valid array (>= 1 element).Num elements = {0}".format(len(rbins)))
+
+
+def convert_to_native_endian(array, warn=False):
+    """The array in native byte order (a swapped copy if needed, with a warning
+    when `warn`); None stays None (Corrfunc/utils.py:866-921)."""
+    import sys
+    import warnings
+    if array is None:
+        return array
+    array = np.asanyarray(array)
+    little = sys.byteorder == "little"
+    if (array.dtype.byteorder == "<") != little and array.dtype.byteorder not in ("=", "|"):
+        if warn:
+            warnings.warn("One or more input array has non-native endianness!  A copy will be made with the "
+                          "correct endianness.")
+        return array.byteswap().view(array.dtype.newbyteorder())
+    return array
+
+
+def is_native_endian(array):
+    """True for None and for arrays in native byte order (Corrfunc/utils.py:924-964)."""
+    import sys
+    if array is None:
+        return True
+    array = np.asanyarray(array)
+    little = sys.byteorder == "little"
+    return array.dtype.byteorder in ("=", "|") or (array.dtype.byteorder == "<") == little
+

@@ -76,3 +76,38 @@ def test_pipeline_on_uniform_points(oracle):
     cf = convert_3d_counts_to_cf(4000, 4000, 12000, 12000, dd["npairs"][1:], dr["npairs"][1:], dr["npairs"][1:],
                                  rr["npairs"][1:])
     assert np.all(np.abs(cf) < 0.05)
+
+
+def test_small_helpers(tmp_path):
+    """translate_isa_string_to_enum, return_file_with_rbins, fix_cz and the
+    endianness helpers behave as the reference's (Corrfunc/utils.py:324-964)."""
+    import os
+    import sys
+    from corrfunc_amd import utils as U
+    assert [U.translate_isa_string_to_enum(k) for k in ("fastest", "FALLBACK", "sse42", "AVX", "avx2", "AVX512F")] \
+        == [-1, 0, 6, 7, 8, 9]
+    with pytest.raises(ValueError):
+        U.translate_isa_string_to_enum("sse5")
+    with pytest.raises(TypeError):
+        U.translate_isa_string_to_enum(3)
+    name, delete = U.return_file_with_rbins([3.0, 1.0, 2.0])
+    assert delete and open(name).read() == "1.0 2.0\n2.0 3.0\n"
+    os.remove(name)
+    p = tmp_path / "b"
+    p.write_text("1 2\n")
+    assert U.return_file_with_rbins(str(p)) == (str(p), False)
+    with pytest.raises(IOError):
+        U.return_file_with_rbins(str(tmp_path / "missing"))
+    with pytest.raises(TypeError):
+        U.return_file_with_rbins([])
+    z = np.array([0.01, 0.02])
+    assert np.allclose(U.fix_cz(z), [2998.0, 5996.0]) and z[0] == 2998.0
+    cz = np.array([3000.0, 5000.0])
+    assert np.array_equal(U.fix_cz(cz), [3000.0, 5000.0])
+    swapped = np.arange(5, dtype=(">" if sys.byteorder == "little" else "<") + "f8")
+    assert not U.is_native_endian(swapped)
+    nat = U.convert_to_native_endian(swapped)
+    assert U.is_native_endian(nat) and np.array_equal(nat, np.arange(5.0))
+    a = np.arange(3.0)
+    assert U.convert_to_native_endian(a) is a and U.convert_to_native_endian(None) is None
+    assert U.is_native_endian(None)
