@@ -53,12 +53,17 @@ def assert_same(mode, got, want, tol=1e-10, what=""):
     assert rel_err(got["weightavg"][s], want["weightavg"][s]) <= tol, (what, mode, "weightavg")
     # xi = ratio - 1 and wp = (ratio - 1) * 2 pimax are compared through the ratio
     # DD / RR, whose relative error is what the sums bound.
+    # (an empty catalog gives NaN in both: compared as equal)
     if "xi" in want:
         a, b = got["xi"][1:], want["xi"][1:]
+        assert np.array_equal(np.isnan(a), np.isnan(b)), (what, mode, "xi NaN")
+        a, b = a[~np.isnan(b)], b[~np.isnan(b)]
         assert np.all(np.abs(a - b) <= tol * (1.0 + np.abs(b))), (what, mode, "xi")
     if "wp" in want:
         t = 2.0 * want["pimax"]
         a, b = got["wp"][1:], want["wp"][1:]
+        assert np.array_equal(np.isnan(a), np.isnan(b)), (what, mode, "wp NaN")
+        a, b = a[~np.isnan(b)], b[~np.isnan(b)]
         assert np.all(np.abs(a - b) <= tol * (t + np.abs(b))), (what, mode, "wp")
 
 
