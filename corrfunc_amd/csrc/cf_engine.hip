@@ -785,8 +785,9 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
             R = r;
         }
         if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
+            // a power of two that fits the workgroup's LDS (others are ignored)
             const int v = atoi(rc);
-            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) R = v;
+            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0 && lds_for(v) <= 64 * 1024) R = v;
         }
         if (E.force_R > 0 && E.force_R < R) R = E.force_R;
         sh.R = R;
