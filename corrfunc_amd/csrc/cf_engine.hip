@@ -92,6 +92,7 @@ struct Engine {
     int sched_kind = CF_AMD_SCHED_LEARNED;
     cf_amd_stats last;
     int ncu = 256;
+    size_t lds_max = 64 * 1024; // LDS one workgroup may allocate (160 KiB on gfx950)
     double ns_per_tick = 10.0;  // device wall clock period
     CostModel model;
     DevLattice hlats[2];
@@ -123,6 +124,11 @@ static Engine& eng()
         hipDeviceProp_t prop;
         CF_CHECK(hipGetDeviceProperties(&prop, E.device));
         E.ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        if (prop.sharedMemPerBlock >= 64 * 1024) E.lds_max = prop.sharedMemPerBlock;
+        if (const char* lm = getenv("CORRFUNC_AMD_LDS_MAX")) {
+            const long v = atol(lm);
+            if (v >= 32 * 1024 && (size_t)v < E.lds_max) E.lds_max = (size_t)v;
+        }
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, E.device) == hipSuccess && khz > 0)
             E.ns_per_tick = 1e6 / khz;
@@ -751,11 +757,14 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         return EXIT_FAILURE;
     }
     // Launch shape for a workgroup of `thr` threads: LDS histogram replication
-    // R (power of 2, <= 32, histograms <= 40 KiB) = the largest that keeps
-    // min(16, best achievable) waves per CU resident; the primaries cache where
-    // it costs no resident workgroup; `per_cu` = resident workgroups per CU
-    // (registers and LDS), which is also how many persistent workgroups per CU
-    // are launched.
+    // R (power of 2, <= 32, histograms <= the LDS limit less 24 KiB) = the
+    // largest that keeps min(16, best achievable) waves per CU resident (on-sky
+    // modes: the best achievable, since their survivors' FP64 decision needs
+    // the latency hiding more than the histogram needs copies; measured on
+    // DDrppi_mocks, R 4 at 16 waves is 15 % slower than R 2 at 24, while
+    // DDrppi gains 4 %); the primaries cache where it costs no resident
+    // workgroup; `per_cu` = resident workgroups per CU (registers and LDS),
+    // which is also how many persistent workgroups per CU are launched.
     struct Shape {
         int threads, nwaves, R, prim, per_cu;
         size_t lds;
@@ -777,17 +786,19 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
             }
             return nb < 1 ? 1 : (nb > 8 ? 8 : nb);
         };
-        const int target = std::min(16, resident(lds_for(1)) * sh.nwaves);
+        const bool onsky = P.mode == MODE_RPPI_MOCKS || P.mode == MODE_SMU_MOCKS;
+        const int best = resident(lds_for(1)) * sh.nwaves;
+        const int target = onsky ? best : std::min(16, best);
         int R = 1;
         for (int r = 2; r <= 32; r *= 2) {
-            if ((size_t)regions * nslots * per_slot * r > 40 * 1024) break;
+            if ((size_t)regions * nslots * per_slot * r > E.lds_max - 24 * 1024) break;
             if (resident(lds_for(r)) * sh.nwaves < target) break;
             R = r;
         }
         if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
             // a power of two that fits the workgroup's LDS (others are ignored)
             const int v = atoi(rc);
-            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0 && lds_for(v) <= 64 * 1024) R = v;
+            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0 && lds_for(v) <= E.lds_max) R = v;
         }
         if (E.force_R > 0 && E.force_R < R) R = E.force_R;
         sh.R = R;
@@ -803,18 +814,21 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     Shape sh = shape_for(256);
     {
         const Shape s8 = shape_for(512);
-        if (s8.lds <= 64 * 1024 && s8.per_cu * s8.nwaves >= sh.per_cu * sh.nwaves) sh = s8;
+        if (s8.lds <= E.lds_max && s8.per_cu * s8.nwaves >= sh.per_cu * sh.nwaves) sh = s8;
         if (const char* wg = getenv("CORRFUNC_AMD_THREADS"))
-            sh = atoi(wg) == 512 && s8.lds <= 64 * 1024 ? s8 : shape_for(256);
+            sh = atoi(wg) == 512 && s8.lds <= E.lds_max ? s8 : shape_for(256);
     }
     const int threads = sh.threads, nwaves = sh.nwaves;
     P.hist_copies = sh.R;
     P.prim_cache = sh.prim;
     const size_t lds = sh.lds;
-    if (lds > 64 * 1024) {
-        fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds 64 KiB\n", lds);
+    if (lds > E.lds_max) {
+        fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds the %zu B a workgroup can have\n",
+                lds, E.lds_max);
         return EXIT_FAILURE;
     }
+    if (lds > 64 * 1024)
+        CF_CHECK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int per_cu = sh.per_cu;
     const long max_blocks = (long)E.ncu * per_cu;
 
