@@ -1892,17 +1892,19 @@ static int sky_distance_table(double zmax, int max_size, double om, double ol, s
     return (int)zc.size();
 }
 
-// Piecewise-linear interpolation in the table; false outside it.
+// Piecewise-linear interpolation in the table (strictly increasing xa); false
+// outside it. The interval is the last lo <= n-2 with xa[lo] <= x (what a
+// bisection finds), reached from the guess of an evenly spaced table by a walk:
+// the redshift table is spaced evenly to ~1e-10, so the walk is 0-1 steps.
 static bool sky_interp(const std::vector<double>& xa, const std::vector<double>& ya, double x, double* y)
 {
     const size_t n = xa.size();
     if (n < 2 || !(x >= xa[0] && x <= xa[n - 1])) return false;
-    size_t lo = 0, hi = n - 1;
-    while (hi > lo + 1) {
-        const size_t m = (hi + lo) / 2;
-        if (xa[m] > x) hi = m;
-        else lo = m;
-    }
+    const double span = xa[n - 1] - xa[0];
+    size_t lo = span > 0.0 ? (size_t)((x - xa[0]) / span * (double)(n - 1)) : 0;
+    if (lo > n - 2) lo = n - 2;
+    while (lo > 0 && xa[lo] > x) lo--;
+    while (lo + 1 <= n - 2 && xa[lo + 1] <= x) lo++;
     const double w = xa[lo + 1] - xa[lo];
     *y = w > 0.0 ? ya[lo] + (x - xa[lo]) / w * (ya[lo + 1] - ya[lo]) : 0.0;
     return true;
@@ -1942,6 +1944,8 @@ static int sky_positions(int64_t n1, const void* ra1, const void* dec1, const vo
         S.x.resize(n);
         S.y.resize(n);
         S.z.resize(n);
+        if (w) S.w.resize(n);
+        else S.w.clear();  // no weights: the caller's size check must not see an earlier call's
         // element-wise, so the result does not depend on the split across host threads
         std::atomic<int64_t> bad{-1};
         auto convert = [&](int64_t lo, int64_t hi) {
@@ -1956,6 +1960,7 @@ static int sky_positions(int64_t n1, const void* ra1, const void* dec1, const vo
                 S.x[i] = d * cd * cos(ra_i * kPiOver180);
                 S.y[i] = d * cd * sin(ra_i * kPiOver180);
                 S.z[i] = d * sin(de * kPiOver180);
+                if (w) S.w[i] = sky_value(w, esize, i);
             }
         };
         const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, n / 65536));
@@ -1968,10 +1973,6 @@ static int sky_positions(int64_t n1, const void* ra1, const void* dec1, const vo
                             "conversion\n", sky_value(cz, esize, bad), zc.empty() ? 0.0 : zc.front() * kSpeedOfLight,
                     zc.empty() ? 0.0 : zc.back() * kSpeedOfLight);
             return EXIT_FAILURE;
-        }
-        if (w) {
-            S.w.resize(n);
-            for (int64_t i = 0; i < n; i++) S.w[i] = sky_value(w, esize, i);
         }
     }
     return EXIT_SUCCESS;
@@ -1993,7 +1994,10 @@ static int sky_run(int mode, int64_t ND1, void* RA1, void* DEC1, void* CZ1, int6
     double om, ol;
     if (sky_cosmology(cosmology, &om, &ol) != EXIT_SUCCESS) return EXIT_FAILURE;
     const int need_w = extra->weight_method != NONE;
-    SkySet s1, s2;
+    // kept across calls (resized, not re-allocated and re-zeroed, when a later
+    // call has the same or fewer points); like the device arena, one per process
+    static SkySet keep[2];
+    SkySet &s1 = keep[0], &s2 = keep[1];
     if (sky_positions(ND1, RA1, DEC1, CZ1, need_w ? extra->weights0.weights[0] : nullptr, ND2, RA2, DEC2, CZ2,
                       need_w && !autocorr ? extra->weights1.weights[0] : nullptr, autocorr, esize,
                       o->is_comoving_dist, cosmology, numthreads, s1, s2) != EXIT_SUCCESS)
