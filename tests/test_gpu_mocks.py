@@ -117,6 +117,28 @@ def test_options(engine, oracle, tmp_path, mode, refine):
         assert_same(mode, got, want, "refine=%s need_avg=%d" % (refine, need_avg))
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_call_sequence(engine, oracle, tmp_path, mode):
+    """Back-to-back calls that grow, shrink and drop weights or the second set:
+    the engine keeps its converted positions between calls, and no call may see
+    an earlier call's points or weights."""
+    _select("tuned")
+    binfile = bins_file(tmp_path, np.array([[0.4, 1.5], [1.5, 4.0], [4.0, 10.0]]))
+    rng = np.random.default_rng(17)
+    seq = [(12000, 9000, True, np.float64), (5000, None, False, np.float64), (7000, None, True, np.float32),
+           (3000, 2000, False, np.float64), (20000, 4000, True, np.float64), (4000, None, True, np.float64)]
+    for k, (n1, n2, weighted, dtype) in enumerate(seq):
+        A = sky_catalog(n1, 100 + k, False, dtype)
+        B = sky_catalog(n2, 200 + k, False, dtype) if n2 else None
+        W = rng.uniform(0.5, 1.5, n1).astype(dtype) if weighted else None
+        W2 = rng.uniform(0.5, 1.5, n2).astype(dtype) if weighted and n2 else None
+        kw = _kw(mode, binfile, A, W, B, W2)
+        got = _abi.call_mocks(engine, mode, options=opts(False), cosmology=1 + k % 2, **kw)
+        want = _abi.call_mocks(oracle, mode, "oracle_", options=opts(False), cosmology=1 + k % 2, **kw)
+        assert_same(mode, got, want, "call %d" % k)
+        assert want["npairs"].sum() > 0
+
+
 def test_ddrppi_mocks_doctest_table_gpu(engine, tmp_path):
     """Corrfunc/mocks/DDrppi_mocks.py:251-317 (uniform points in a 420 box around
     the observer, distances given)."""
