@@ -442,11 +442,18 @@ def call_mocks(lib, mode, prefix="", *, autocorr=1, nthreads=1, binfile=None, RA
                mu_max=None, nmu_bins=None, cosmology=1, options=None):
     """Run on-sky entry point `mode` ("countpairs_mocks" or "countpairs_mocks_s_mu")
     of `lib`; returns a dict of numpy arrays as call_mode does, or None on failure.
-    The position arrays handed to C are copies (the library may fix them in place)."""
+    The caller's arrays are not changed: where the library would fix a set in
+    place (RA < 0, DEC > 90, or cz read as z), that set is handed over as a copy."""
     (RA1, DEC1, CZ1, RA2, DEC2, CZ2), dt = _prep_positions([RA1, DEC1, CZ1, RA2, DEC2, CZ2])
-    RA1, DEC1, CZ1 = [a.copy() for a in (RA1, DEC1, CZ1)]
+    comoving = bool(options.is_comoving_dist) if options is not None else False
+
+    def _private(ra, dec, cz):
+        if len(ra) and (ra.min() < 0 or dec.max() > 90 or (not comoving and cz.max() < 10)):
+            return [a.copy() for a in (ra, dec, cz)]
+        return [ra, dec, cz]
+    RA1, DEC1, CZ1 = _private(RA1, DEC1, CZ1)
     if RA2 is not None:
-        RA2, DEC2, CZ2 = [a.copy() for a in (RA2, DEC2, CZ2)]
+        RA2, DEC2, CZ2 = _private(RA2, DEC2, CZ2)
     n1 = len(RA1)
     n2 = len(RA2) if RA2 is not None else 0
     method = WEIGHT_NONE

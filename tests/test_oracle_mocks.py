@@ -183,8 +183,12 @@ def test_input_fixes_and_cosmologies(oracle, tmp_path):
                            options=opts(False), **kw)
     # redshifts (max < 10) are taken for z and multiplied by c; RA in [-180, 180)
     # is shifted by 180 and DEC in [0, 180] by -90: shift the inputs the other way
-    fixed = _abi.call_mocks(oracle, "countpairs_mocks", "oracle_", RA1=ra - 180.0, DEC1=dec + 90.0, CZ1=z,
+    shifted = [ra - 180.0, dec + 90.0, z.copy()]
+    keep = [a.copy() for a in shifted]
+    fixed = _abi.call_mocks(oracle, "countpairs_mocks", "oracle_", RA1=shifted[0], DEC1=shifted[1], CZ1=shifted[2],
                             cosmology=1, options=opts(False), **kw)
+    for a, k in zip(shifted, keep):  # the library fixes a copy, not the caller's arrays
+        assert np.array_equal(a, k)
     s = _abi.reported_slots("countpairs_mocks", base)
     assert base["npairs"][s].sum() > 0
     tot_b, tot_f = int(base["npairs"][s].sum()), int(fixed["npairs"][s].sum())
