@@ -27,7 +27,10 @@ def test_many_small_items_tuned_equals_plain(engine):
     X = uniform(1048576, L, 42, np.float64)
     bins = np.logspace(-1, np.log10(25.0), 21)
     got, st = _call(engine, _lib.ENGINE_TUNED, X, bins, L)
-    want, _ = _call(engine, _lib.ENGINE_PLAIN, X, bins, L)
+    try:
+        want, _ = _call(engine, _lib.ENGINE_PLAIN, X, bins, L)
+    finally:
+        _lib.set_engine(_lib.ENGINE_TUNED)
     # the list this catalog gives is in the batched regime
     assert st["num_work_items"] <= 8 * st["num_cell_pairs"]
     assert st["num_work_items"] >= 100 * 256 * 24  # 24 resident waves per CU at most
