@@ -1,7 +1,8 @@
 #!/usr/bin/env python
 """Pair-counting throughput on one MI355X: pairs tested per second.
 
-Five workloads (--workload, default 2); one "step" is one pass of the hot path
+Five workloads (--workload, default 5: the largest single-GPU workload by pairs
+tested per step); one "step" is one pass of the hot path
 over the workload's synthetic input, which is generated with numpy and copied
 to HBM before timing (the timed region is the C-ABI calls on HBM-resident
 inputs: gridding, work list, pair kernel, histogram reduction, finalisation):
@@ -22,7 +23,9 @@ inputs: gridding, work list, pair kernel, histogram reduction, finalisation):
      records on a second catalog of the same recipe.
   5  DDsmu cross-correlation: 2,000,000 clustered points (the workload-2
      recipe) x 20,000,000 uniform randoms, float32, periodic L = 1000, 30
-     linear s bins 0-150, 40 mu bins to mu_max = 1, pair_product weights.
+     linear s bins 0-150, 40 mu bins to mu_max = 1, pair_product weights;
+     the work list is ordered by the learned cost model, trained (untimed) on
+     timings the engine records on a second clustered catalog of the recipe.
 
 "Pairs tested" is the reference's unit of work: the sum over its neighbour-cell
 pair list (default options) of N_i * N_j (N(N-1)/2 for a cell with itself),
@@ -47,16 +50,17 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# The pair filter is 32-bit integer vector work: 256 CUs x 64 lanes x 2.4 GHz =
-# 39.3e12 lane-instructions/s, at most 4 ops each (a packed dot product: 2 mul +
-# 2 add), i.e. 157.3 TOP/s.
-PEAK_VALU_TOPS = 157.3
-# integer ops of the pair filter per tested pair: 3-D: 3 sub, 3 mul, 2 add, the
-# lower-bound offset add, 1 compare; projected: 3 sub, 2 mul, 1 add, the offset
-# add, the z offset add, 2 compares
-OPS_PER_PAIR = {"countpairs": 10, "countpairs_xi": 10, "countpairs_s_mu": 10,
-                "countpairs_rp_pi": 10, "countpairs_wp": 10, "countpairs_mocks": 10,
-                "countpairs_mocks_s_mu": 10}
+# Roofline of the pair kernels: algorithmic FP32 work of the distance test per
+# tested pair (3-D: 3 sub + 3 mul + 2 add for r^2; projected: 2 sub + 2 mul +
+# 1 add for rp^2 and 1 sub for dz), against MI355X's FP32 vector peak
+# (256 CUs x 128 lanes x 2 (FMA) x 2.4 GHz = 157.3 TFLOP/s). The plain FP64
+# form is also priced against the FP64 vector peak, 78.6 TFLOP/s.
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+FLOPS_PER_PAIR = {"countpairs": 8, "countpairs_xi": 8, "countpairs_s_mu": 8,
+                  "countpairs_rp_pi": 6, "countpairs_wp": 6, "countpairs_mocks": 8,
+                  "countpairs_mocks_s_mu": 8}
+SCHED_NAMES = {0: "none", 1: "heuristic", 2: "learned"}
 METRIC = "pairs tested/s on 1 MI355X: tuned vs plain GPU form, npairs bit-exact vs FP64"
 
 
@@ -124,6 +128,27 @@ def cpu_reference():
     return _abi.bind(C.CDLL(so))
 
 
+def median_time(fn, runs):
+    """Runs fn() `runs` times; returns (median seconds, every time, last result)."""
+    ts, r = [], None
+    for _ in range(max(1, runs)):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts, r
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu / /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def npairs_equal(mode, a, b):
     from corrfunc_amd import _abi
     s = _abi.reported_slots(mode, a)
@@ -145,9 +170,12 @@ class Workload:
     def plain_step(self):
         return self.step()
 
-    def cpu(self, cores, gpu_outs, tested):
-        """Times the reference on the host; returns (seconds, pairs tested in the
-        timed sample, sample description, counts equal to the GPU's) or None."""
+    plain_steps = 0     # default plain steps (0: max(1, steps // 2))
+
+    def cpu(self, cores, gpu_outs, tested, runs):
+        """Times the reference on the host `runs` times; returns (median seconds,
+        pairs tested in the timed sample, sample description, counts equal to the
+        GPU's[, kind]) or None."""
         return None
 
 
@@ -169,16 +197,14 @@ class W1(Workload):
         out, st = self.call()
         return [out], [st]
 
-    def cpu(self, cores, gpu_outs, tested):
+    def cpu(self, cores, gpu_outs, tested, runs):
         from corrfunc_amd import _abi
         ref = cpu_reference()
         if ref is None:
             return None
-        t0 = time.perf_counter()
-        r = _abi.call_mode(ref, "countpairs", options=_opts(periodic=True, boxsize=420.0, need_avg_sep=True,
-                                                            isa="fastest"),
-                           autocorr=1, nthreads=cores, binfile=self.bins, X1=self.X[0], Y1=self.X[1], Z1=self.X[2])
-        dt = time.perf_counter() - t0
+        dt, _, r = median_time(lambda: _abi.call_mode(
+            ref, "countpairs", options=_opts(periodic=True, boxsize=420.0, need_avg_sep=True, isa="fastest"),
+            autocorr=1, nthreads=cores, binfile=self.bins, X1=self.X[0], Y1=self.X[1], Z1=self.X[2]), runs)
         return dt, tested, "the full workload (one call), isa=fastest", npairs_equal("countpairs", r, gpu_outs[0])
 
 
@@ -202,22 +228,24 @@ class W2(Workload):
         b, sb = self.wp()
         return [a, b], [sa, sb]
 
-    def cpu(self, cores, gpu_outs, tested):
+    def cpu(self, cores, gpu_outs, tested, runs):
         from corrfunc_amd import _abi
         ref = cpu_reference()
         if ref is None:
             return None
         X64 = [a.astype(np.float64) for a in self.X]
         W64 = self.W.astype(np.float64)
-        t0 = time.perf_counter()
-        r1 = _abi.call_mode(ref, "countpairs_rp_pi", options=_opts(periodic=True, boxsize=1000.0, need_avg_sep=True,
-                                                                   isa="fastest"),
-                            autocorr=1, nthreads=cores, binfile=self.bins, X1=X64[0], Y1=X64[1], Z1=X64[2],
-                            weights1=W64, weight_type="pair_product", pimax=40.0)
-        r2 = _abi.call_mode(ref, "countpairs_wp", options=_opts(need_avg_sep=True, isa="fastest"), nthreads=cores,
-                            binfile=self.bins, X1=X64[0], Y1=X64[1], Z1=X64[2], weights1=W64,
-                            weight_type="pair_product", pimax=40.0, boxsize=1000.0)
-        dt = time.perf_counter() - t0
+
+        def one():
+            r1 = _abi.call_mode(ref, "countpairs_rp_pi", options=_opts(periodic=True, boxsize=1000.0,
+                                                                       need_avg_sep=True, isa="fastest"),
+                                autocorr=1, nthreads=cores, binfile=self.bins, X1=X64[0], Y1=X64[1], Z1=X64[2],
+                                weights1=W64, weight_type="pair_product", pimax=40.0)
+            r2 = _abi.call_mode(ref, "countpairs_wp", options=_opts(need_avg_sep=True, isa="fastest"),
+                                nthreads=cores, binfile=self.bins, X1=X64[0], Y1=X64[1], Z1=X64[2], weights1=W64,
+                                weight_type="pair_product", pimax=40.0, boxsize=1000.0)
+            return r1, r2
+        dt, _, (r1, r2) = median_time(one, runs)
         ok = npairs_equal("countpairs_rp_pi", r1, gpu_outs[0]) and npairs_equal("countpairs_wp", r2, gpu_outs[1])
         return dt, tested, "the full workload, one DDrppi + one wp call (one step), FP64 path on up-cast inputs, " \
                         "isa=fastest", ok
@@ -263,28 +291,33 @@ class W3(Workload):
             stats.append(_lib.last_stats())
         return outs, stats
 
-    def cpu(self, cores, gpu_outs, tested):
+    plain_steps = 2
+
+    def cpu(self, cores, gpu_outs, tested, runs):
         # bounded sample: the smallest catalogs, in order, until ~2e10 pairs tested
         from corrfunc_amd import _abi
         ref = cpu_reference()
         if ref is None:
             return None
         order = np.argsort([len(c[0]) for c in self.cats])
-        t_sum, ok, frac_pairs, k = 0.0, True, 0.0, 0
         per = self._tested_per_catalog()
         tested_total = sum(per)
-        for k, i in enumerate(order):
-            c = self.cats[i]
-            t0 = time.perf_counter()
-            r = _abi.call_mode(ref, "countpairs", options=_opts(periodic=True, boxsize=500.0, need_avg_sep=True,
-                                                                isa="fastest"),
-                               autocorr=1, nthreads=cores, binfile=self.bins,
-                               X1=c[0].astype(np.float64), Y1=c[1].astype(np.float64), Z1=c[2].astype(np.float64))
-            t_sum += time.perf_counter() - t0
-            ok = ok and npairs_equal("countpairs", r, gpu_outs[i])
+        sample, frac_pairs = [], 0.0
+        for i in order:
+            sample.append(i)
             frac_pairs += per[i] / tested_total
             if frac_pairs * tested_total >= 2e10:
                 break
+        cats64 = {i: [a.astype(np.float64) for a in self.cats[i]] for i in sample}
+
+        def one():
+            return [_abi.call_mode(ref, "countpairs", options=_opts(periodic=True, boxsize=500.0,
+                                                                    need_avg_sep=True, isa="fastest"),
+                                   autocorr=1, nthreads=cores, binfile=self.bins, X1=cats64[i][0],
+                                   Y1=cats64[i][1], Z1=cats64[i][2]) for i in sample]
+        t_sum, _, rs = median_time(one, runs)
+        ok = all(npairs_equal("countpairs", r, gpu_outs[i]) for r, i in zip(rs, sample))
+        k = len(sample) - 1
         return t_sum, frac_pairs * tested_total, "the %d smallest catalogs (%.2f%% of the pairs tested), sequential reference " \
                                   "calls, isa=fastest" % (k + 1, 100 * frac_pairs), ok
 
@@ -330,32 +363,48 @@ class W4(Workload):
         out, st = self.call()
         return [out], [st]
 
-    def cpu(self, cores, gpu_outs, tested):
+    def cpu(self, cores, gpu_outs, tested, runs):
         from corrfunc_amd import _abi
         ref = cpu_reference()
         if ref is None:
             return None
         X64 = [a.astype(np.float64) for a in self.X]
-        t0 = time.perf_counter()
-        r = _abi.call_mode(ref, "countpairs_xi", options=_opts(need_avg_sep=True, isa="fastest"), nthreads=cores,
-                           binfile=self.bins, X1=X64[0], Y1=X64[1], Z1=X64[2], boxsize=1000.0)
-        dt = time.perf_counter() - t0
+        dt, _, r = median_time(lambda: _abi.call_mode(
+            ref, "countpairs_xi", options=_opts(need_avg_sep=True, isa="fastest"), nthreads=cores,
+            binfile=self.bins, X1=X64[0], Y1=X64[1], Z1=X64[2], boxsize=1000.0), runs)
         return dt, tested, "the full workload (one call), isa=fastest", npairs_equal("countpairs_xi", r, gpu_outs[0])
 
 
 class W5(Workload):
     name = "smu_cross_2M_20M"
     dominant = "countpairs_s_mu"
-    data = "synthetic (numpy default_rng seed 1005)"
+    data = "synthetic (numpy default_rng seeds 1005/10050 (measured) and 1015 (cost-model training))"
+    plain_steps = 2
 
     def __init__(self, args):
-        from corrfunc_amd import catalogs
+        from corrfunc_amd import catalogs, scheduler
         nd = args.n or 2000000
         nr = 10 * nd
-        self.D, self.WD = catalogs.neyman_scott(n=nd, L=1000.0, nparent=2000, sigma=2.0, seed=1005)
         rng = np.random.default_rng(10050)
         self.R, self.WR = catalogs.uniform32(nr, 1000.0, rng, with_weights=True)
         self.bins = np.linspace(0.0, 150.0, 31)
+        if args.sched == "learned":
+            # cost model: trained on timings recorded on a second clustered
+            # catalog of the recipe against the same randoms (untimed)
+            D2, W2 = catalogs.neyman_scott(n=nd, L=1000.0, nparent=2000, sigma=2.0, seed=1015)
+            tc = Call("countpairs_s_mu", D2, W2, self.R, self.WR, autocorr=0, bins=self.bins, L=1000.0,
+                      mu_max=1.0, nmu_bins=40)
+            scheduler.clear()
+            scheduler.record(True)
+            tc()
+            scheduler.record(False)
+            Xs, ys = scheduler.samples()
+            t0 = time.perf_counter()
+            self.train_loss = scheduler.train(Xs, ys, epochs=20, batch=512, lr=3e-3, seed=0)
+            self.train_s = time.perf_counter() - t0
+            self.train_n = len(ys)
+            del tc, D2, W2
+        self.D, self.WD = catalogs.neyman_scott(n=nd, L=1000.0, nparent=2000, sigma=2.0, seed=1005)
         self.call = Call("countpairs_s_mu", self.D, self.WD, self.R, self.WR, autocorr=0, bins=self.bins, L=1000.0,
                          mu_max=1.0, nmu_bins=40)
         self.description = ("DDsmu cross-correlation, %d clustered (Neyman-Scott) x %d uniform float32, periodic "
@@ -365,7 +414,7 @@ class W5(Workload):
         out, st = self.call()
         return [out], [st]
 
-    def cpu(self, cores, gpu_outs, tested):
+    def cpu(self, cores, gpu_outs, tested, runs):
         # bounded sample: every 64th data point against all randoms, on the CPU
         # and (untimed) on the GPU, whose counts must agree
         from corrfunc_amd import _abi
@@ -385,10 +434,9 @@ class W5(Workload):
         kw64 = dict(kw)
         for k in ("X1", "Y1", "Z1", "X2", "Y2", "Z2", "weights1", "weights2"):
             kw64[k] = kw[k].astype(np.float64)
-        t0 = time.perf_counter()
-        r = _abi.call_mode(ref, "countpairs_s_mu", options=_opts(periodic=True, boxsize=1000.0, need_avg_sep=True,
-                                                                 isa="fastest"), nthreads=cores, **kw64)
-        dt = time.perf_counter() - t0
+        dt, _, r = median_time(lambda: _abi.call_mode(
+            ref, "countpairs_s_mu", options=_opts(periodic=True, boxsize=1000.0, need_avg_sep=True, isa="fastest"),
+            nthreads=cores, **kw64), runs)
         return dt, pairs, "every 64th data point (%d) x all %d randoms, isa=fastest" % (len(sub[0]), len(self.R[0])), \
             npairs_equal("countpairs_s_mu", r, g)
 
@@ -422,7 +470,7 @@ class W6(Workload):
             raise RuntimeError("countpairs_mocks failed")
         return [out], [_lib.last_stats()]
 
-    def cpu(self, cores, gpu_outs, tested):
+    def cpu(self, cores, gpu_outs, tested, runs):
         # the reference's mocks engines need GSL, absent here: the oracle (a C
         # restatement with OpenMP) is the CPU baseline for this workload
         from corrfunc_amd import _abi
@@ -430,9 +478,7 @@ class W6(Workload):
         if not os.path.isfile(so):
             return None
         ora = _abi.bind(C.CDLL(so), "oracle_")
-        t0 = time.perf_counter()
-        r = self._call(ora, "oracle_")
-        dt = time.perf_counter() - t0
+        dt, _, r = median_time(lambda: self._call(ora, "oracle_"), runs)
         return dt, tested, "the full workload (one call), the oracle's C restatement, fallback order", \
             npairs_equal("countpairs_mocks", r, gpu_outs[0]), "port"
 
@@ -445,10 +491,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", type=int, default=2, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", type=int, default=5, choices=sorted(WORKLOADS))
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=0, help="default: max(1, steps // 2)")
+    ap.add_argument("--plain-steps", type=int, default=0,
+                    help="default: 2 for workloads 3 and 5 (seconds per plain step), else max(1, steps // 2)")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU-baseline runs (the median is reported)")
     ap.add_argument("--n", type=int, default=0, help="override the workload's point count (smaller test runs)")
     ap.add_argument("--sched", default="learned", choices=["none", "heuristic", "learned"])
     ap.add_argument("--sched-ab", action="store_true", help="also time the pair kernel under every scheduler")
@@ -517,18 +565,29 @@ def main():
 
     plain = parity_plain = None
     if not args.no_plain:
-        ps = args.plain_steps or max(1, args.steps // 2)
+        ps = args.plain_steps or wl.plain_steps or max(1, args.steps // 2)
         t_plain, out_p, st_p = timed(_lib.ENGINE_PLAIN, ps, 0 if args.workload in (3, 5) else 1, wl.plain_step)
         plain = tested * ps / t_plain
         parity_plain = all(np.array_equal(a["npairs"], b["npairs"]) for a, b in zip(out_p, out_t))
         _lib.set_engine(_lib.ENGINE_TUNED)
         print("plain: %.4g pairs tested/s" % plain, file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel: the tuned pair kernel of the workload's main call
+    # roofline of the dominant kernel: the tuned pair kernel of the workload's
+    # main call; its duration comes from HIP events the engine records around
+    # the launch on its own stream
     dom = [s for s in st_t[0]][0]
     k_ms = float(np.mean([s[0]["kernel_ms"] for s in st_t]))
     dom_pairs = dom["tested_pairs"]
-    achieved = dom_pairs * OPS_PER_PAIR[wl.dominant] / (k_ms * 1e-3) / 1e12
+    flops = FLOPS_PER_PAIR[wl.dominant]
+    achieved = dom_pairs * flops / (k_ms * 1e-3) / 1e12
+    plain_roof = None
+    if plain is not None:
+        pk_ms = float(np.mean([s[0]["kernel_ms"] for s in st_p]))
+        pa = dom_pairs * flops / (pk_ms * 1e-3) / 1e12
+        plain_roof = {"bound": "valu", "kernel": "k_pairs_plain", "achieved": pa, "peak": PEAK_FP64_TFLOPS,
+                      "unit": "TFLOP/s", "frac": pa / PEAK_FP64_TFLOPS, "kernel_ms": pk_ms,
+                      "algorithmic": "%d FP64 flops per tested pair x %.4g tested pairs per launch"
+                                     % (flops, dom_pairs)}
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_k_pairs_tuned_%s.txt" % wl.name)
     if os.path.isfile(tpath):
@@ -575,13 +634,15 @@ def main():
 
     cpu = parity_cpu = None
     if not args.no_cpu:
-        cores = min(os.cpu_count() or 1, 16)
-        r = wl.cpu(cores, out_t, tested)
+        # every host core this process may run on (OpenMP threads of the reference)
+        cores = len(os.sched_getaffinity(0))
+        r = wl.cpu(cores, out_t, tested, args.cpu_runs)
         if r is not None:
             dt, pairs, sample, parity_cpu = r[:4]
             kind = r[4] if len(r) > 4 else "reference"
             cpu = {"value": pairs / dt, "unit": "pairs tested/s", "cores": cores, "kind": kind,
-                   "sample": "%s, %.1f s" % (sample, dt)}
+                   "cpu_model": cpu_model(),
+                   "sample": "%s; median of %d runs, %.2f s" % (sample, max(1, args.cpu_runs), dt)}
 
     line = {
         "metric": METRIC,
@@ -597,26 +658,31 @@ def main():
         "dtype": wl.dtype,
         "data": wl.data,
         "config": {"workload": wl.description, "tested_pairs_per_step": tested, "parallelism": "single GPU",
-                   "scheduler": args.sched, "autotune": bool(args.autotune),
+                   # the scheduler that ordered the dominant call's work list (the
+                   # engine's own statistic), and the one asked for
+                   "scheduler": SCHED_NAMES[st_t[-1][0]["sched"]], "scheduler_requested": args.sched,
+                   "autotune": bool(args.autotune),
                    "engine_tested_pairs_per_step": engine_tested,
                    "refine": list(st_t[-1][0]["refine"]), "autotuned": st_t[-1][0]["autotuned"],
-                   "hist_copies": st_t[-1][0]["hist_copies"]},
+                   "hist_copies": st_t[-1][0]["hist_copies"], "queue_fetch": st_t[-1][0]["queue_fetch"],
+                   "global_hist": st_t[-1][0]["global_hist"]},
         "plain_value": plain,
         "tuned_over_plain": (value / plain) if plain else None,
         "useful_pairs_per_s": useful,
         "parity": {"tuned_vs_plain_npairs_equal": parity_plain, "gpu_vs_cpu_reference_npairs_equal": parity_cpu},
         "roofline": {"bound": "valu", "kernel": "k_pairs_tuned<%s>" % wl.dominant, "achieved": achieved,
-                     "peak": PEAK_VALU_TOPS, "unit": "TOP/s", "frac": achieved / PEAK_VALU_TOPS,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic, "kernel_ms": k_ms,
-                     "algorithmic": "%d integer ops per tested pair x %.4g tested pairs per launch"
-                                    % (OPS_PER_PAIR[wl.dominant], dom_pairs)},
+                     "algorithmic": "%d FP32 flops per tested pair x %.4g tested pairs per launch"
+                                    % (flops, dom_pairs)},
+        "plain_roofline": plain_roof,
         "cpu_baseline": cpu,
     }
     if ab is not None:
         line["scheduler_ab"] = ab
     if sweep is not None:
         line["sweep"] = {args.sweep.split("=", 1)[0]: sweep}
-    if args.workload == 4 and hasattr(wl, "train_loss"):
+    if hasattr(wl, "train_loss"):
         line["cost_model"] = {"train_samples": wl.train_n, "train_loss": wl.train_loss, "train_s": wl.train_s}
     print(json.dumps(line))
 

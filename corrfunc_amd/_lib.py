@@ -16,7 +16,8 @@ class cf_amd_stats(C.Structure):
     _fields_ = [("nmesh", C.c_int * 3), ("refine", C.c_int * 3), ("num_cell_pairs", C.c_int64),
                 ("num_work_items", C.c_int64), ("tested_pairs", C.c_double), ("kernel_ms", C.c_double),
                 ("grid_ms", C.c_double), ("total_ms", C.c_double), ("engine", C.c_int), ("sched", C.c_int),
-                ("list_ms", C.c_double), ("hist_copies", C.c_int), ("autotuned", C.c_int)]
+                ("list_ms", C.c_double), ("hist_copies", C.c_int), ("autotuned", C.c_int),
+                ("queue_fetch", C.c_int), ("global_hist", C.c_int)]
 
 
 ENGINE_TUNED = 0
@@ -100,7 +101,8 @@ def last_stats():
     return dict(nmesh=tuple(s.nmesh), refine=tuple(s.refine), num_cell_pairs=s.num_cell_pairs,
                 num_work_items=s.num_work_items, tested_pairs=s.tested_pairs, kernel_ms=s.kernel_ms,
                 grid_ms=s.grid_ms, total_ms=s.total_ms, engine=s.engine, sched=s.sched, list_ms=s.list_ms,
-                hist_copies=s.hist_copies, autotuned=s.autotuned)
+                hist_copies=s.hist_copies, autotuned=s.autotuned, queue_fetch=s.queue_fetch,
+                global_hist=s.global_hist)
 
 
 def set_autotune(on=True):

@@ -91,6 +91,9 @@ struct PairParams {
     float dense_frac;        // a cell pair whose estimated in-range fraction reaches this takes the dense path
     int fetch;               // work items per queue atomic outside the single-fetch ranges
     int prim_cache;          // 1: the sparse path keeps the wave's primaries in LDS for its drains
+    int fix_move_bits;       // fixed-point words of magnitude >= 2^fix_move_bits move to global memory (61)
+    int fix_check_calls;     // histogram updates between overflow checks, per workgroup (2^13)
+    int max_window;          // longest secondary window scanned with one queue base (2^25, a multiple of 4)
 };
 
 // Work item of the tuned kernel: (cell pair, <= 64 primary particles, one part

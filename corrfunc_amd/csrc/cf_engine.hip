@@ -18,6 +18,9 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -673,11 +676,61 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         P.dense_stride = ds ? atoi(ds) : 1;
         const char* df = getenv("CORRFUNC_AMD_DENSE_FRAC");
         P.dense_frac = df ? (float)atof(df) : 0.35f;
+        // overflow moves of the fixed-point histogram sums (k_pairs_tuned): the
+        // variables exist to exercise the move path in tests
+        const char* fm = getenv("CORRFUNC_AMD_FIX_MOVE_BITS");
+        P.fix_move_bits = fm ? std::max(1, std::min(61, atoi(fm))) : 61;
+        const char* fc = getenv("CORRFUNC_AMD_FIX_CHECK_CALLS");
+        P.fix_check_calls = fc ? std::max(1, std::min(1 << 13, atoi(fc))) : 1 << 13;
+        // sub-window length of the sparse scan (tests shorten it to exercise the
+        // window changes; a multiple of 4 below 2^26)
+        const char* mw = getenv("CORRFUNC_AMD_MAX_WINDOW");
+        P.max_window = mw ? std::max(4, std::min(1 << 25, atoi(mw))) & ~3 : 1 << 25;
         const char* fe = getenv("CORRFUNC_AMD_FETCH");
         P.fetch = fe ? std::max(1, atoi(fe)) : 0;  // 0: chosen at launch from the work list
         if (eng().force_fetch > 0) P.fetch = eng().force_fetch;
     }
 
+}
+
+// What the pair kernels read as `rupp_sqr`: the squared edges [nbin], the
+// power-of-2 scales of the fixed-point separation sums per bin [nbin] and the
+// weight-product scale [1] (written on the device by weight_scale_device; 0
+// here). The scale of bin k makes the largest separation a pair of bin k can
+// have -- the largest edge at or above k, rmax for sorted edges -- at most
+// 2^42 fixed-point units.
+static std::vector<double> edge_table(const std::vector<double>& rupp, const std::vector<double>& rupp_sqr, int nbin)
+{
+    std::vector<double> h(2 * (size_t)nbin + 1, 0.0);
+    double hi = 0.0;
+    for (int k = nbin - 1; k >= 0; k--) {
+        h[k] = rupp_sqr[k];
+        if (std::isfinite(rupp[k])) hi = fmax(hi, rupp[k]);
+        double sc = 1.0;
+        if (hi > 0.0) {
+            int ex = 0;
+            frexp(hi, &ex);
+            sc = ldexp(1.0, std::max(-1000, std::min(1000, 42 - ex)));
+        }
+        h[nbin + k] = sc;
+    }
+    return h;
+}
+
+// Scale of the fixed-point weight-product sums from max|w| of the sets
+// (k_weight_scale), written to *d_wsc on the stream.
+static void weight_scale_device(const DevLattice* sets, int nsets, bool same, double* d_wsc)
+{
+    Engine& E = eng();
+    unsigned long long* m = (unsigned long long*)E.arena.get("wmax", 2 * sizeof(unsigned long long));
+    k_fill_u64<<<1, 64, 0, E.stream>>>(2, m, 0ull);
+    for (int k = 0; k < nsets; k++) {
+        const DevLattice& L = sets[k];
+        unsigned long long* mk = (same || k == 0) ? m : m + 1;
+        if (L.n > 0 && L.w) k_absmax<<<(unsigned)std::min<int64_t>(1024, (L.n + 255) / 256), 256, 0, E.stream>>>(L.n, L.w, mk);
+    }
+    k_weight_scale<<<1, 64, 0, E.stream>>>(m, same ? m : m + 1, d_wsc);
+    CF_CHECK(hipGetLastError());
 }
 
 // Device copy of the (primary, secondary) lattice pair of a single call.
@@ -715,8 +768,14 @@ constexpr int kBatchItemsPerWave = 100;
 constexpr double kBatchItemsPerCellPair = 8.0;
 constexpr int kSplitPerWave = 4;  // a work item is split when it exceeds 1/(4 * resident waves) of the total cost
 
+// launch_tuned's answer when even a one-copy LDS layout (edges, lookup,
+// queues) does not fit a workgroup: the call is counted by the plain kernel
+constexpr int kNoTunedFit = 2;
+
 struct TunedRun {
     int nitems = 0;
+    int fetch = 0;                        // work items per queue atomic
+    int global_hist = 0;                  // 1: histogram in global memory (too large for LDS)
     unsigned long long* ticks = nullptr;  // per cell pair wall-clock ticks (device), when timed
     int* cu = nullptr;                    // compute unit of the last item of each cell pair (device)
 };
@@ -746,15 +805,29 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
 {
     Engine& E = eng();
     const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
+    // LDS of everything but the histogram for a workgroup of `nw` waves:
+    // squared edges and sum scales, bin lookup, survivor queues
+    auto lds_fixed_for = [&](int nw) {
+        return (size_t)nbin * 16 + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nw * kQueueAlloc * 4;
+    };
+    // a histogram whose single copy does not fit next to the rest of a 4-wave
+    // workgroup is kept in global memory (no replication; FP64 atomics)
+    const bool gh = !batch && (size_t)nslots * per_slot + lds_fixed_for(4) > E.lds_max;
+    if (lds_fixed_for(4) > E.lds_max || (batch && (size_t)nslots * per_slot * 4 + lds_fixed_for(4) > E.lds_max))
+        return kNoTunedFit;
     const void* kfn = nullptr;
-#define CF_K(M, F, B) if (P.mode == M && P.r2form == F && batch == B) kfn = (const void*)&k_pairs_tuned<M, F, B>;
+#define CF_K(M, F, B)                                                                                          \
+    if (P.mode == M && P.r2form == F && batch == B)                                                            \
+        kfn = gh ? (const void*)&k_pairs_tuned<M, F, B, true> : (const void*)&k_pairs_tuned<M, F, B, false>;
     CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
     CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
     CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
     CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
     CF_K(MODE_RPPI_MOCKS, R2_SUM_XY_Z, 0) CF_K(MODE_SMU_MOCKS, R2_SUM_XY_Z, 0)
-    CF_K(MODE_DD, R2_SUM_XY_Z, 1) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 1)
 #undef CF_K
+    if (batch && P.mode == MODE_DD && P.r2form == R2_SUM_XY_Z) kfn = (const void*)&k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true, false>;
+    if (batch && P.mode == MODE_DD && P.r2form == R2_FMA_DZ_DY_DX)
+        kfn = (const void*)&k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true, false>;
     if (!kfn) {
         fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d%s\n", P.mode, P.r2form,
                 batch ? " (batched)" : "");
@@ -777,9 +850,8 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         Shape sh;
         sh.threads = thr;
         sh.nwaves = thr / 64;
-        const int regions = batch ? sh.nwaves : 1;
-        const size_t lds_fixed =
-            (size_t)nbin * 8 + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)sh.nwaves * kQueueAlloc * 4;
+        const int regions = gh ? 0 : (batch ? sh.nwaves : 1);
+        const size_t lds_fixed = lds_fixed_for(sh.nwaves);
         const size_t lds_prim = 16 + (size_t)sh.nwaves * 64 * sizeof(double4);
         auto lds_for = [&](int r) { return (size_t)regions * nslots * per_slot * r + lds_fixed; };
         auto resident = [&](size_t l) {
@@ -794,7 +866,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         const int best = resident(lds_for(1)) * sh.nwaves;
         const int target = onsky ? best : std::min(16, best);
         int R = 1;
-        for (int r = 2; r <= 32; r *= 2) {
+        for (int r = 2; r <= 32 && !gh; r *= 2) {
             if ((size_t)regions * nslots * per_slot * r > E.lds_max - 24 * 1024) break;
             if (resident(lds_for(r)) * sh.nwaves < target) break;
             R = r;
@@ -802,7 +874,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         if (const char* rc = getenv("CORRFUNC_AMD_HIST_COPIES")) {
             // a power of two that fits the workgroup's LDS (others are ignored)
             const int v = atoi(rc);
-            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0 && lds_for(v) <= E.lds_max) R = v;
+            if (!gh && v >= 1 && v <= 64 && (v & (v - 1)) == 0 && lds_for(v) <= E.lds_max) R = v;
         }
         if (E.force_R > 0 && E.force_R < R) R = E.force_R;
         sh.R = R;
@@ -939,6 +1011,8 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         CF_CHECK(hipGetLastError());
     }
     T.nitems = nitems;
+    T.fetch = P.fetch;
+    T.global_hist = gh ? 1 : 0;
     T.ticks = ticks;
     T.cu = cu;
     return EXIT_SUCCESS;
@@ -1279,34 +1353,46 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     unsigned long long* g_np = (unsigned long long*)E.arena.get("h_np", sizeof(uint64_t) * nslots);
     double* g_avg = (double*)E.arena.get("h_avg", sizeof(double) * nslots);
     double* g_w = (double*)E.arena.get("h_w", sizeof(double) * nslots);
-    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * nbin);
-    CF_CHECK(hipMemsetAsync(g_np, 0, sizeof(uint64_t) * nslots, E.stream));
-    CF_CHECK(hipMemsetAsync(g_avg, 0, sizeof(double) * nslots, E.stream));
-    CF_CHECK(hipMemsetAsync(g_w, 0, sizeof(double) * nslots, E.stream));
-    CF_CHECK(hipMemcpyAsync(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice, E.stream));
+    const std::vector<double> etab = edge_table(R.rupp, rupp_sqr, nbin);
+    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * etab.size());
+    k_fill_u64<<<(unsigned)((nslots + 255) / 256), 256, 0, E.stream>>>(nslots, g_np, 0ull);
+    k_fill_u64<<<(unsigned)((nslots + 255) / 256), 256, 0, E.stream>>>(nslots, (unsigned long long*)g_avg, 0ull);
+    k_fill_u64<<<(unsigned)((nslots + 255) / 256), 256, 0, E.stream>>>(nslots, (unsigned long long*)g_w, 0ull);
+    CF_CHECK(hipMemcpyAsync(d_edges, etab.data(), sizeof(double) * etab.size(), hipMemcpyHostToDevice, E.stream));
+    if (need_w) {
+        const DevLattice sets[2] = {L1, L2};
+        weight_scale_device(sets, c.autocorr ? 1 : 2, c.autocorr != 0, d_edges + 2 * nbin);
+    }
     unsigned short* d_lut = (unsigned short*)E.arena.get("lut", sizeof(unsigned short) * lut.size());
     CF_CHECK(hipMemcpyAsync(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice, E.stream));
-    const size_t lds_hist = (size_t)nslots * 24 + (size_t)nbin * 8;
-    if (lds_hist + 4096 > 64 * 1024) {
-        fprintf(stderr, "Error: %lld histogram slots exceed the engine's LDS histogram capacity\n", (long long)nslots);
-        return EXIT_FAILURE;
-    }
 
     int nitems = 0;
     TunedRun T;
-    // the tuned kernel packs secondary indices into 26 bits; larger secondary
-    // sets are counted by the plain kernel
-    const bool tuned = ncp > 0 && E.engine_kind != CF_AMD_ENGINE_PLAIN && L2.n < kMaxSecondary;
+    bool tuned = ncp > 0 && E.engine_kind != CF_AMD_ENGINE_PLAIN;
     if (tuned) {
-        if (launch_tuned(ncp, cps, lats_for(L1, L2), 0, P, (int)nslots, nbin, d_edges, d_lut, g_np, g_avg, g_w,
-                         c.cell_timer || E.model.recording, T) != EXIT_SUCCESS)
+        const int rc = launch_tuned(ncp, cps, lats_for(L1, L2), 0, P, (int)nslots, nbin, d_edges, d_lut, g_np,
+                                    g_avg, g_w, c.cell_timer || E.model.recording, T);
+        if (rc == kNoTunedFit) {
+            // more edges than a workgroup's LDS holds next to the queues: plain kernel
+            tuned = false;
+        } else if (rc != EXIT_SUCCESS) {
             return EXIT_FAILURE;
+        }
         nitems = T.nitems;
-    } else {
+    }
+    if (!tuned) {
+        // the plain kernel keeps an FP64 LDS histogram when it fits 64 KiB,
+        // else adds to the global histogram
+        const size_t lds_hist = (size_t)nslots * 24 + (size_t)nbin * 8;
+        const bool plain_gh = lds_hist + 4096 > 64 * 1024;
+        T.global_hist = plain_gh ? 1 : 0;
         CF_CHECK(hipEventRecord(E.ev_list, E.stream));
         CF_CHECK(hipEventRecord(E.ev0, E.stream));
         if (ncp > 0) {
-            k_pairs_plain<<<ncp, 128, lds_hist, E.stream>>>(P, L1, L2, cps, d_edges, g_np, g_avg, g_w);
+            if (plain_gh)
+                k_pairs_plain<true><<<ncp, 128, 0, E.stream>>>(P, L1, L2, cps, d_edges, g_np, g_avg, g_w);
+            else
+                k_pairs_plain<false><<<ncp, 128, lds_hist, E.stream>>>(P, L1, L2, cps, d_edges, g_np, g_avg, g_w);
             CF_CHECK(hipGetLastError());
         }
         CF_CHECK(hipEventRecord(E.ev1, E.stream));
@@ -1343,7 +1429,9 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     CF_CHECK(hipEventElapsedTime(&lms, E.ev_list, E.ev0));
 
     st.list_ms = lms;
-    st.hist_copies = tuned ? P.hist_copies : 0;
+    st.hist_copies = tuned && !T.global_hist ? P.hist_copies : 0;
+    st.queue_fetch = tuned ? T.fetch : 0;
+    st.global_hist = T.global_hist;
     st.autotuned = E.last_autotuned;
     st.sched = tuned ? (E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC
                                                                                   : E.sched_kind)
@@ -1356,7 +1444,8 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     st.kernel_ms = kms;
     st.grid_ms = (t_grid - t_start) * 1e3;
     st.total_ms = (now_s() - t_start) * 1e3;
-    st.engine = E.engine_kind;
+    // the plain kernel also runs where the tuned one cannot (kNoTunedFit)
+    st.engine = ncp > 0 && !tuned ? CF_AMD_ENGINE_PLAIN : E.engine_kind;
     E.last = st;
     return EXIT_SUCCESS;
 }
@@ -1409,11 +1498,6 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         const std::string tag = "b" + std::to_string(k);
         if (need_w && W[k] == nullptr) {
             fprintf(stderr, "Error: catalog %d has no weights\n", k);
-            return EXIT_FAILURE;
-        }
-        if (ND[k] >= kMaxSecondary) {
-            fprintf(stderr, "Error: catalog %d has %lld particles; the batched kernel takes fewer than %lld\n", k,
-                    (long long)ND[k], (long long)kMaxSecondary);
             return EXIT_FAILURE;
         }
         dx[k] = to_device(tag + "x", X[k], ND[k], esize);
@@ -1547,20 +1631,28 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     unsigned long long* g_np = (unsigned long long*)E.arena.get("bh_np", sizeof(uint64_t) * tot);
     double* g_avg = (double*)E.arena.get("bh_avg", sizeof(double) * tot);
     double* g_w = (double*)E.arena.get("bh_w", sizeof(double) * tot);
-    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * nbin);
+    const std::vector<double> etab = edge_table(rupp, rupp_sqr, nbin);
+    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * etab.size());
     unsigned short* d_lut = (unsigned short*)E.arena.get("lut", sizeof(unsigned short) * lut.size());
-    CF_CHECK(hipMemsetAsync(g_np, 0, sizeof(uint64_t) * tot, E.stream));
-    CF_CHECK(hipMemsetAsync(g_avg, 0, sizeof(double) * tot, E.stream));
-    CF_CHECK(hipMemsetAsync(g_w, 0, sizeof(double) * tot, E.stream));
-    CF_CHECK(hipMemcpyAsync(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice, E.stream));
+    const unsigned fb = (unsigned)std::min<size_t>(4096, (tot + 255) / 256);
+    k_fill_u64<<<fb, 256, 0, E.stream>>>((int64_t)tot, g_np, 0ull);
+    k_fill_u64<<<fb, 256, 0, E.stream>>>((int64_t)tot, (unsigned long long*)g_avg, 0ull);
+    k_fill_u64<<<fb, 256, 0, E.stream>>>((int64_t)tot, (unsigned long long*)g_w, 0ull);
+    CF_CHECK(hipMemcpyAsync(d_edges, etab.data(), sizeof(double) * etab.size(), hipMemcpyHostToDevice, E.stream));
+    if (need_w) weight_scale_device(hl.data(), ncat, true, d_edges + 2 * nbin);
     CF_CHECK(hipMemcpyAsync(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice, E.stream));
 
     int nitems = 0;
     TunedRun T;
     if (ncp_all > 0) {
-        if (launch_tuned(ncp_all, cps, lats, 1, P, nslots, nbin, d_edges, d_lut, g_np, g_avg, g_w,
-                         E.model.recording, T) != EXIT_SUCCESS)
+        const int rc = launch_tuned(ncp_all, cps, lats, 1, P, nslots, nbin, d_edges, d_lut, g_np, g_avg, g_w,
+                                    E.model.recording, T);
+        if (rc == kNoTunedFit) {
+            fprintf(stderr, "Error: %d bins are more than the batched kernel's LDS holds; count the catalogs with "
+                            "countpairs one at a time\n", nbin - 1);
             return EXIT_FAILURE;
+        }
+        if (rc != EXIT_SUCCESS) return EXIT_FAILURE;
         nitems = T.nitems;
     } else {
         CF_CHECK(hipEventRecord(E.ev_list, E.stream));
@@ -1608,6 +1700,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     st.engine = CF_AMD_ENGINE_TUNED;
     st.list_ms = lms;
     st.hist_copies = P.hist_copies;
+    st.queue_fetch = T.fetch;
     st.autotuned = E.last_autotuned;
     st.sched = E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC : E.sched_kind;
     E.last = st;
@@ -1807,14 +1900,32 @@ static void finalize(CallOut& R, int autocorr, int64_t ND1, int need_avg, int se
     }
 }
 
+// Every entry point runs under one process-wide lock: the engine's device
+// buffers, stream and host copies are shared by all calls, so calls from
+// several threads are serialised (the reference's functions are not
+// re-entrant either; its Python extension holds the GIL's place with the
+// caller's). No exception leaves an extern "C" function.
+static std::recursive_mutex& engine_lock()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
 template <typename F>
 static int guarded(F&& f)
 {
+    std::lock_guard<std::recursive_mutex> hold(engine_lock());
     try {
         return f();
     } catch (const HipFailure&) {
         return EXIT_FAILURE;
     } catch (const InputError&) {
+        return EXIT_FAILURE;
+    } catch (const std::bad_alloc&) {
+        fprintf(stderr, "corrfunc_amd: out of host memory\n");
+        return EXIT_FAILURE;
+    } catch (const std::exception& ex) {
+        fprintf(stderr, "corrfunc_amd: %s\n", ex.what());
         return EXIT_FAILURE;
     }
 }
@@ -2365,7 +2476,8 @@ int countpairs_wp(const int64_t ND1, void* X1, void* Y1, void* Z1, const double 
             rlow = results->rupp[i];
         }
         reset_bin_refine_factors(options);
-        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        // wp reports nanoseconds (countpairs_wp_impl.c.src:669-673); the other modes seconds
+        if (options->c_api_timer) options->c_api_time = (now_s() - t0) * 1e9;
         return EXIT_SUCCESS;
     });
 }

@@ -414,6 +414,43 @@ __global__ void k_cellpair_emit(NgbArgs a, int64_t ncells, int nngb, const int* 
     }
 }
 
+// ------------------------------------------- scales of the fixed-point sums
+// max |w| of a weight array as the bit pattern of a non-negative double
+// (non-negative doubles order like their bits; NaN orders above +inf), max-ed
+// into *out (zeroed by the caller).
+__global__ void k_absmax(int64_t n, const double* __restrict__ w, unsigned long long* __restrict__ out)
+{
+    unsigned long long m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(w[i]));
+        m = b > m ? b : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(m, o, 64);
+        m = t > m ? t : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+// Power-of-2 scale of the weight-product sums: max|w1| * max|w2| * scale <= 2^41.
+// 0 (sums in FP64 in global memory) when a weight is not finite or the
+// products are too small for a normal scale; 1 when every weight is 0.
+__global__ void k_weight_scale(const unsigned long long* __restrict__ m1, const unsigned long long* __restrict__ m2,
+                               double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double m = __longlong_as_double((long long)*m1) * __longlong_as_double((long long)*m2);
+    double s = 0.0;
+    if (m == 0.0) {
+        s = 1.0;
+    } else if (isfinite(m)) {
+        int ex = 0;
+        frexp(m, &ex);
+        if (41 - ex < 1000) s = ldexp(1.0, 41 - ex);
+    }
+    *out = s;
+}
+
 // ------------------------------------------------- exact pair replay (a8-a12)
 __device__ __forceinline__ double r2_3d(int f, double dx, double dy, double dz)
 {
@@ -616,7 +653,10 @@ __device__ void hist_init(const PairParams& P, LdsHist& h)
 
 // -------------------------------------------------- plain FP64 pair kernel
 // One workgroup per cell pair; every (i, j) of the pair is decided in FP64 with
-// the full reference predicate. This is the "plain GPU form".
+// the full reference predicate. This is the "plain GPU form". Its histogram is
+// FP64 in LDS (GH = false), or the global histogram itself (GH = true) when
+// the slots do not fit the workgroup's LDS.
+template <bool GH>
 __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1, DevLattice L2,
                                                      const CellPairRec* __restrict__ cps,
                                                      const double* __restrict__ rupp_sqr,
@@ -625,13 +665,23 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
 {
     extern __shared__ unsigned long long smem_u64[];
     LdsHist h;
-    h.np = smem_u64;
-    h.avg = reinterpret_cast<double*>(smem_u64 + P.nslots);
-    h.w = h.avg + P.nslots;
-    double* e = h.w + P.nslots;
-    for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
-    hist_init(P, h);
-    __syncthreads();
+    const double* e;
+    if (GH) {
+        // no LDS: histogram and edges in global memory
+        h.np = g_np;
+        h.avg = g_avg;
+        h.w = g_w;
+        e = rupp_sqr;
+    } else {
+        h.np = smem_u64;
+        h.avg = reinterpret_cast<double*>(smem_u64 + P.nslots);
+        h.w = h.avg + P.nslots;
+        double* el = h.w + P.nslots;
+        hist_init(P, h);
+        for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) el[s] = rupp_sqr[s];
+        e = el;
+        __syncthreads();
+    }
     const CellPairRec cp = cps[blockIdx.x];
     const int mode = P.mode;
     const int i0 = L1.bstart[(int64_t)cp.c1 * kSlabs], i1 = L1.bstart[(int64_t)(cp.c1 + 1) * kSlabs];
@@ -648,9 +698,14 @@ __global__ __launch_bounds__(128) void k_pairs_plain(PairParams P, DevLattice L1
             if (slot >= 0) hist_add(P, h, slot, sep, P.need_w ? wi * L2.w[j] : 0.0);
         }
     }
+    if (GH) return;
     __syncthreads();
     hist_flush(P, h, g_np, g_avg, g_w);
 }
+template __global__ void k_pairs_plain<false>(PairParams, DevLattice, DevLattice, const CellPairRec*, const double*,
+                                              unsigned long long*, double*, double*);
+template __global__ void k_pairs_plain<true>(PairParams, DevLattice, DevLattice, const CellPairRec*, const double*,
+                                             unsigned long long*, double*, double*);
 
 // ------------------------------------------------- tuned pair kernel
 // Work item: (cell pair, chunk of <= 64 consecutive primary particles).
@@ -678,8 +733,11 @@ constexpr int kBlockJ = 4;          // secondaries per scalar load of the sparse
 // remaining < 64 entries move to the front.
 constexpr int kCompactAt = 128;
 constexpr int kQueueAlloc = kCompactAt + 64 + 64 * (kBlockJ - 1) + 127;
-// Secondary indices must fit the packed queue entry ((j << 6) | lane, 32 bits).
-constexpr int64_t kMaxSecondary = (int64_t)1 << 26;
+// Queue entries hold secondary indices relative to the start of the window
+// being scanned, ((j - jbase) << 6) | lane in 32 bits: a secondary window longer
+// than P.max_window (2^25 by default, a multiple of kBlockJ) is scanned in
+// consecutive sub-windows of that length, with the queue drained at the end of
+// each.
 
 // Read-only particle data seen through the constant address space: loads with a
 // wave-uniform index become scalar loads.
@@ -695,17 +753,136 @@ __device__ __forceinline__ uint2 kload_u2(KUint* p, int j)
 }
 typedef short short2v __attribute__((ext_vector_type(2)));
 
-// LDS histogram replicated R ways with layout [slot][R]: lanes of one wave that
-// land in the same slot add to different copies, which sit in different banks.
+// sqrt(r2) to ~1e-14 relative: the hardware FP32 square root (about 1 ulp)
+// refined by one Newton step in FP64.
+__device__ __forceinline__ double fast_sqrt(double r2)
+{
+    const float f = (float)r2;
+    if (!(f > 1e-30f)) return sqrt(r2);
+    const float s = __builtin_amdgcn_sqrtf(f);
+    const double sd = (double)s;
+    const double res = fma(-sd, sd, r2);
+    const double half_inv = 0.5 * (double)__builtin_amdgcn_rcpf(s);
+    return fma(res, half_inv, sd);
+}
+
+// Histogram of one workgroup of the tuned kernel, replicated R ways with layout
+// [slot][R]: lanes of one wave that land in the same slot add to different
+// copies, which sit in different banks. Counts are u64; the sums of
+// separations and of weight products are exact fixed-point integers (fix_add),
+// so every update is a 64-bit integer LDS atomic (the FP64 LDS atomic costs
+// about 20x more per update on gfx950). GH kernels (histograms larger than
+// the workgroup's LDS) add to the global FP64 histogram instead.
 struct KernelLds {
     unsigned long long* np;
-    double* avg;
-    double* w;
+    long long* avg;            // fixed point, scale sc[bin]
+    long long* w;              // fixed point, scale wsc
     int rmask;                 // R - 1
     int lane_copy;             // lane & (R - 1)
     const double* e;           // rupp_sqr[0..nbin-1]
+    const double* sc;          // [nbin] power-of-2 scale of the separation sums per bin
     const unsigned short* lut; // bin lookup
+    double wsc;                // power-of-2 scale of the weight-product sums; 0: FP64 in global memory
+    unsigned long long* gnp;   // the global histogram this region flushes to
+    double* gavg;
+    double* gw;
+    int calls;                 // histogram updates (of <= 64 lanes) since the last overflow check
+    int calls_lim;
 };
+
+// Fixed-point sums. An addend x becomes q = round(x * scale) with scale a power
+// of 2 chosen so that |q| <= 2^43 (sc[k] * rupp[k] <= 2^42 for the separations of
+// bin k; wsc * max|w1| * max|w2| <= 2^41). One FP64 FMA with 1.5 * 2^52 puts
+// round(x * scale) in the low mantissa bits (the sum lies in [2^52, 2^53), where
+// the spacing of doubles is 1). Addends with |q| < 2^37 (x below 1/16-1/32 of the
+// bin's upper edge, or tiny weights) are added in FP64 to the global histogram
+// instead, so every fixed-point addend has a relative rounding error below
+// 0.5 / 2^37 = 3.6e-12 and the sums agree with FP64 sums to that bound.
+// Overflow: a wave checks its workgroup's region every calls_lim updates (of
+// <= 64 addends each, calls_lim = 2^13 / waves) and moves every word with
+// |value| >= 2^61 to the global histogram; between two checks of a word by any
+// wave, the waves add less than 2^19 * 2^43 = 2^62 to it, so no word leaves int64.
+// (The move threshold 2^61 and the check interval 2^13 are PairParams fields,
+// fix_move_bits and fix_check_calls, so that tests can force frequent moves.)
+constexpr long long kFixEscape = 1LL << 37;
+constexpr long long kFixMagicBits = 0x4338000000000000LL;  // bits of 1.5 * 2^52
+
+__device__ __forceinline__ long long fix_q(double x, double scale)
+{
+    return __double_as_longlong(fma(x, scale, 0x1.8p52)) - kFixMagicBits;
+}
+
+__device__ __forceinline__ void fix_add(long long* p, double* gp, double x, double scale)
+{
+    const long long q = fix_q(x, scale);
+    if (q < kFixEscape && q > -kFixEscape && x != 0.0)
+        atomicAdd(gp, x);
+    else
+        atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)q);
+}
+
+// histogram bin (rp / s / r bin) of a slot
+__device__ __forceinline__ int slot_bin(const PairParams& P, int slot) { return P.n2 > 0 ? slot / (P.n2 + 1) : slot; }
+
+// Moves every fixed-point word of the wave's region whose magnitude reached
+// kFixMove to the global histogram (one wave, all lanes; other waves may be
+// adding to the same words: the move is an atomic exchange with 0).
+__device__ void fix_check(const PairParams& P, KernelLds& S)
+{
+    const int R = S.rmask + 1;
+    const int n = P.nslots * R;
+    const long long kFixMove = 1LL << P.fix_move_bits;
+    for (int t = lane_id(); t < n; t += 64) {
+        if (P.need_avg) {
+            const long long v = __hip_atomic_load(&S.avg[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (v >= kFixMove || v <= -kFixMove) {
+                const long long o = (long long)atomicExch(reinterpret_cast<unsigned long long*>(&S.avg[t]), 0ull);
+                atomicAdd(&S.gavg[t / R], (double)o / S.sc[slot_bin(P, t / R)]);
+            }
+        }
+        if (P.need_w && S.wsc != 0.0) {
+            const long long v = __hip_atomic_load(&S.w[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (v >= kFixMove || v <= -kFixMove) {
+                const long long o = (long long)atomicExch(reinterpret_cast<unsigned long long*>(&S.w[t]), 0ull);
+                atomicAdd(&S.gw[t / R], (double)o / S.wsc);
+            }
+        }
+    }
+}
+
+// Counts one histogram update of the wave (wave-uniform control flow) and runs
+// the overflow check when due.
+template <bool GH>
+__device__ __forceinline__ void fix_tick(const PairParams& P, KernelLds& S)
+{
+    if (GH) return;
+    if (++S.calls >= S.calls_lim) {
+        S.calls = 0;
+        fix_check(P, S);
+    }
+}
+
+// Adds a counted pair (slot, bin k, squared separation r2, weight product) to
+// the histogram.
+template <bool GH>
+__device__ __forceinline__ void hist_pair(const PairParams& P, KernelLds& S, int slot, int k, double r2, double wprod)
+{
+    if (GH) {
+        atomicAdd(&S.gnp[slot], 1ull);
+        if (P.need_avg) atomicAdd(&S.gavg[slot], fast_sqrt(r2));
+        if (P.need_w) atomicAdd(&S.gw[slot], wprod);
+        return;
+    }
+    const int a = slot * (S.rmask + 1) + S.lane_copy;
+    atomicAdd(&S.np[a], 1ull);
+    if (P.need_avg) fix_add(&S.avg[a], &S.gavg[slot], fast_sqrt(r2), S.sc[k]);
+    if (P.need_w) {
+        if (S.wsc != 0.0)
+            fix_add(&S.w[a], &S.gw[slot], wprod, S.wsc);
+        else
+            atomicAdd(&S.gw[slot], wprod);
+    }
+}
 
 // Number of edges e[0..nbin-2] <= r2 (the reference's top-down bin search,
 // countpairs_kernels.c.src:982-993, for non-decreasing edges): start from the
@@ -718,19 +895,6 @@ __device__ __forceinline__ int lut_kbin(const PairParams& P, const KernelLds& S,
     int k = S.lut[idx];
     while (k < P.nbin - 1 && S.e[k] <= r2) k++;
     return k;
-}
-
-// sqrt(r2) to ~1e-14 relative: the hardware FP32 square root (about 1 ulp)
-// refined by one Newton step in FP64.
-__device__ __forceinline__ double fast_sqrt(double r2)
-{
-    const float f = (float)r2;
-    if (!(f > 1e-30f)) return sqrt(r2);
-    const float s = __builtin_amdgcn_sqrtf(f);
-    const double sd = (double)s;
-    const double res = fma(-sd, sd, r2);
-    const double half_inv = 0.5 * (double)__builtin_amdgcn_rcpf(s);
-    return fma(res, half_inv, sd);
 }
 
 // FP64 separation in the rounding order R2F of the reference kernel being replayed.
@@ -766,7 +930,7 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* fresh_k(co
 // Decides one (i, j) pair in FP64 exactly as the reference does and adds it to
 // the LDS histogram. (xpos, ypos, zpos) = primary position plus the cell pair's
 // wrap; (xj, yj, zj) = secondary position. `ok` masks lanes without a pair.
-template <int MODE, int R2F>
+template <int MODE, int R2F, bool GH>
 __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRec* cpp, const DevLattice* l1p, int i,
                                            int ok, double xpos, double ypos, double zpos, double wi, double xj,
                                            double yj, double zj, double wj, KernelLds& S)
@@ -788,10 +952,7 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
         }
         const int k = lut_kbin(P, S, key);
         if (k == 0) return;
-        const int a = (k * (P.n2 + 1) + j2) * (S.rmask + 1) + S.lane_copy;
-        atomicAdd(&S.np[a], 1ull);
-        if (P.need_avg) atomicAdd(&S.avg[a], fast_sqrt(key));
-        if (P.need_w) atomicAdd(&S.w[a], wi * wj);
+        hist_pair<GH>(P, S, k * (P.n2 + 1) + j2, k, key, wi * wj);
         return;
     }
     const double sqr_dz = dz * dz;
@@ -849,22 +1010,21 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
             slot = k * (P.n2 + 1) + mb;
         }
     }
-    const int a = slot * (S.rmask + 1) + S.lane_copy;
-    atomicAdd(&S.np[a], 1ull);
-    if (P.need_avg) atomicAdd(&S.avg[a], fast_sqrt(r2));
-    if (P.need_w) atomicAdd(&S.w[a], wi * wj);
+    hist_pair<GH>(P, S, slot, k, r2, wi * wj);
 }
 
-template <int MODE, int R2F>
+// Decides up to 64 queued survivors (entries ((j - jbase) << 6) | lane) in FP64.
+template <int MODE, int R2F, bool GH>
 __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const CellPairRec* cpp,
                                             const DevLattice* l1p, const double4* d4_1, const double4* d4_2,
                                             const double3& wrap, int i0, int istride, const unsigned* qe,
-                                            const double4* prim, unsigned head, int ndrain, KernelLds& S)
+                                            const double4* prim, unsigned head, int ndrain, int jbase,
+                                            KernelLds& S)
 {
     const int lane = lane_id();
     if (lane < ndrain) {
         const unsigned e = qe[head + lane];
-        const int j = (int)(e >> 6), l = (int)(e & 63);
+        const int j = jbase + (int)(e >> 6), l = (int)(e & 63);
         const int i = i0 + l * istride;
         double4 pi;
         if (P.prim_cache) {
@@ -876,8 +1036,9 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
             pi.z += wrap.z;
         }
         const double4 pj = d4_2[j];
-        exact_pair<MODE, R2F>(P, cpp, l1p, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
+        exact_pair<MODE, R2F, GH>(P, cpp, l1p, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
+    fix_tick<GH>(P, S);
 }
 
 // Integer pair filter on quantized cell-local positions (secondary s, primary
@@ -948,10 +1109,10 @@ __device__ __forceinline__ QBlock qload(KUint* qs, int jb)
 }
 
 // Filters block b (secondaries jb .. jb + kBlockJ - 1, of which the first
-// `nlive` exist) and enqueues the survivors.
+// `nlive` exist) and enqueues the survivors relative to jbase.
 template <int MODE, bool SAME, bool PARTIAL>
 __device__ __forceinline__ void qblock(const QFilter& f, const QBlock& b, int jb, int nlive, unsigned* qe,
-                                       unsigned& tail)
+                                       unsigned& tail, int jbase)
 {
     unsigned long long m[kBlockJ];
 #pragma unroll
@@ -959,7 +1120,7 @@ __device__ __forceinline__ void qblock(const QFilter& f, const QBlock& b, int jb
         m[t] = qpass<MODE, SAME>(f, b.s[t], jb + t);
         if (PARTIAL) m[t] = t < nlive ? m[t] : 0ull;
     }
-    const unsigned e0 = ((unsigned)jb << 6) | (unsigned)lane_id();
+    const unsigned e0 = ((unsigned)(jb - jbase) << 6) | (unsigned)lane_id();
 #pragma unroll
     for (int t = 0; t < kBlockJ; t++)
         if (m[t]) enqueue(qe, m[t], tail, e0 + 64u * t);
@@ -972,27 +1133,27 @@ __device__ __forceinline__ void qblock(const QFilter& f, const QBlock& b, int jb
 // by kQPad entries, so the look-ahead stays inside the array).
 template <int MODE, bool SAME>
 __device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jhi, QBlock& a, unsigned* qe,
-                                     unsigned head, unsigned& tail)
+                                     unsigned head, unsigned& tail, int jbase)
 {
     int nfull = (jhi - jb) / kBlockJ;
     for (;;) {
         if (nfull <= 0 || tail - head >= 64) return jb;
         const QBlock b = qload(qs, jb + kBlockJ);
-        qblock<MODE, SAME, false>(f, a, jb, kBlockJ, qe, tail);
+        qblock<MODE, SAME, false>(f, a, jb, kBlockJ, qe, tail, jbase);
         jb += kBlockJ;
         if (--nfull <= 0 || tail - head >= 64) {
             a = b;
             return jb;
         }
         a = qload(qs, jb + kBlockJ);
-        qblock<MODE, SAME, false>(f, b, jb, kBlockJ, qe, tail);
+        qblock<MODE, SAME, false>(f, b, jb, kBlockJ, qe, tail, jbase);
         jb += kBlockJ;
         --nfull;
     }
 }
 
 // One work item of the tuned kernel (see the comment above kQueues).
-template <int MODE, int R2F>
+template <int MODE, int R2F, bool GH>
 __device__ __forceinline__ void process_item(const PairParams& P, const DevLattice* l1p, const DevLattice* l2p,
                                              const CellPairRec& cp, const CellPairRec* cpp, const WorkItem& it,
                                              KernelLds& S, unsigned* qe, double4* prim)
@@ -1068,8 +1229,9 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         double4 cur = kload_d4(ds, jlo);
         for (int j = jlo; j < jhi; j++) {
             const double4 nxt = kload_d4(ds, j + 1);
-            exact_pair<MODE, R2F>(P, cpp, l1p, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
-                                  P.need_w ? cur.w : 0.0, S);
+            exact_pair<MODE, R2F, GH>(P, cpp, l1p, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
+                                      P.need_w ? cur.w : 0.0, S);
+            fix_tick<GH>(P, S);
             cur = nxt;
         }
         return;
@@ -1100,49 +1262,57 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         __builtin_amdgcn_wave_barrier();
     }
     KUint* qs = (KUint*)(const void*)L2.q;
-    QBlock a = qload(qs, jlo);
-    int jb = jlo;
-    while (jb < jhi) {
-        if (same)
-            jb = qscan<MODE, true>(f, qs, jb, jhi, a, qe, head, tail);
-        else
-            jb = qscan<MODE, false>(f, qs, jb, jhi, a, qe, head, tail);
-        if (jb < jhi && jb + kBlockJ > jhi && tail - head < 64) {
-            // the last, partial block
+    const int max_window = Pi.max_window;
+    for (int wlo = jlo; wlo < jhi; wlo += max_window) {
+        const int whi = jhi - wlo > max_window ? wlo + max_window : jhi;
+        QBlock a = qload(qs, wlo);
+        int jb = wlo;
+        while (jb < whi) {
             if (same)
-                qblock<MODE, true, true>(f, a, jb, jhi - jb, qe, tail);
+                jb = qscan<MODE, true>(f, qs, jb, whi, a, qe, head, tail, wlo);
             else
-                qblock<MODE, false, true>(f, a, jb, jhi - jb, qe, tail);
-            jb = jhi;
+                jb = qscan<MODE, false>(f, qs, jb, whi, a, qe, head, tail, wlo);
+            if (jb < whi && jb + kBlockJ > whi && tail - head < 64) {
+                // the last, partial block
+                if (same)
+                    qblock<MODE, true, true>(f, a, jb, whi - jb, qe, tail, wlo);
+                else
+                    qblock<MODE, false, true>(f, a, jb, whi - jb, qe, tail, wlo);
+                jb = whi;
+            }
+            while (tail - head >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head, 64, wlo,
+                                           S);
+                __builtin_amdgcn_wave_barrier();
+                head += 64;
+            }
+            if (head >= kCompactAt) {
+                // move the < 64 waiting entries to the front
+                const unsigned n = tail - head;
+                unsigned v = 0;
+                if ((unsigned)lane < n) v = qe[head + lane];
+                __builtin_amdgcn_wave_barrier();
+                if ((unsigned)lane < n) qe[lane] = v;
+                __builtin_amdgcn_wave_barrier();
+                head = 0;
+                tail = n;
+            }
         }
-        while (tail - head >= 64) {
+        if (tail != head) {
             __builtin_amdgcn_wave_barrier();
-            drain_queue<MODE, R2F>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head, 64, S);
+            drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                                       (int)(tail - head), wlo, S);
             __builtin_amdgcn_wave_barrier();
-            head += 64;
         }
-        if (head >= kCompactAt) {
-            // move the < 64 waiting entries to the front
-            const unsigned n = tail - head;
-            unsigned v = 0;
-            if ((unsigned)lane < n) v = qe[head + lane];
-            __builtin_amdgcn_wave_barrier();
-            if ((unsigned)lane < n) qe[lane] = v;
-            __builtin_amdgcn_wave_barrier();
-            head = 0;
-            tail = n;
-        }
-    }
-    if (tail != head) {
-        __builtin_amdgcn_wave_barrier();
-        drain_queue<MODE, R2F>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head, (int)(tail - head),
-                               S);
-        __builtin_amdgcn_wave_barrier();
+        head = tail = 0;
     }
 }
 
 // Adds a (replicated) LDS histogram region to the global histogram at `base`
-// and clears it; `t0`/`tstep` split the slots among the calling threads.
+// and clears it; `t0`/`tstep` split the slots among the calling threads. The
+// fixed-point sums of the R copies are converted to FP64 (exact integers below
+// 2^63, scaled by a power of 2) and summed in FP64.
 __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, int R, int64_t base, int t0,
                                              int tstep, unsigned long long* g_np, double* g_avg, double* g_w)
 {
@@ -1152,13 +1322,13 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
         for (int r = 0; r < R; r++) {
             c += S.np[sl * R + r];
             S.np[sl * R + r] = 0;
-            if (P.need_avg) { sa += S.avg[sl * R + r]; S.avg[sl * R + r] = 0; }
-            if (P.need_w) { sw += S.w[sl * R + r]; S.w[sl * R + r] = 0; }
+            if (P.need_avg) { sa += (double)S.avg[sl * R + r]; S.avg[sl * R + r] = 0; }
+            if (P.need_w) { sw += (double)S.w[sl * R + r]; S.w[sl * R + r] = 0; }
         }
         if (c) {
             atomicAdd(&g_np[base + sl], c);
-            if (P.need_avg) atomicAdd(&g_avg[base + sl], sa);
-            if (P.need_w) atomicAdd(&g_w[base + sl], sw);
+            if (P.need_avg) atomicAdd(&g_avg[base + sl], sa / S.sc[slot_bin(P, sl)]);
+            if (P.need_w && S.wsc != 0.0) atomicAdd(&g_w[base + sl], sw / S.wsc);
         }
     }
 }
@@ -1167,8 +1337,11 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
 // in a batch (BATCH = true) every cell pair carries its catalog index k in
 // `pad`, both sets are lats[k], each wave keeps its own LDS histogram for the
 // catalog it is working on, and the histogram of catalog k lives at
-// g_*[k*nslots ...].
-template <int MODE, int R2F, bool BATCH>
+// g_*[k*nslots ...]. GH = true: no LDS histogram; every counted pair is added
+// to the global FP64 histogram (histograms too large for the workgroup's LDS).
+// rupp_sqr holds the squared edges [nbin], the per-bin scales of the
+// separation sums [nbin] and the weight-product scale [1] (fix_add).
+template <int MODE, int R2F, bool BATCH, bool GH>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
                                                      const WorkItem* __restrict__ items,
@@ -1182,24 +1355,32 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
 {
     extern __shared__ unsigned long long smem_u64[];
     KernelLds S;
-    const int R = P.hist_copies;
+    const int R = GH ? 1 : P.hist_copies;
     const int nwaves = blockDim.x >> 6;
-    const int nrep = P.nslots * R;                  // one histogram region
+    const int nrep = GH ? 0 : P.nslots * R;         // one histogram region
     const int nreg = BATCH ? nrep * nwaves : nrep;  // all regions of the workgroup
     const int wave = threadIdx.x >> 6, lane = lane_id();
     unsigned long long* np0 = smem_u64;
-    double* avg0 = reinterpret_cast<double*>(smem_u64 + nreg);
-    double* w0 = avg0 + (P.need_avg ? nreg : 0);
-    double* e = w0 + (P.need_w ? nreg : 0);
+    long long* avg0 = reinterpret_cast<long long*>(smem_u64 + nreg);
+    long long* w0 = avg0 + (P.need_avg ? nreg : 0);
+    double* e = reinterpret_cast<double*>(w0 + (P.need_w ? nreg : 0));
+    double* sc = e + P.nbin;
     const int roff = BATCH ? wave * nrep : 0;
     S.np = np0 + roff;
     S.avg = avg0 + roff;
     S.w = w0 + roff;
     S.rmask = R - 1;
     S.lane_copy = lane & (R - 1);
-    unsigned short* lut = reinterpret_cast<unsigned short*>(e + P.nbin);
+    S.wsc = rupp_sqr[2 * P.nbin];
+    S.gnp = g_np;
+    S.gavg = g_avg;
+    S.gw = g_w;
+    S.calls = 0;
+    S.calls_lim = max(1, P.fix_check_calls / (BATCH ? 1 : nwaves));
+    unsigned short* lut = reinterpret_cast<unsigned short*>(sc + P.nbin);
     int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
     S.e = e;
+    S.sc = sc;
     S.lut = lut;
     unsigned* qe = reinterpret_cast<unsigned*>(qbase) + __builtin_amdgcn_readfirstlane(wave) * kQueueAlloc;
     // primaries cache: 16-byte aligned after the queues
@@ -1207,7 +1388,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const size_t poff = ((size_t)(reinterpret_cast<char*>(qbase + nwaves * kQueueAlloc) -
                                   reinterpret_cast<char*>(smem_u64)) + 15) & ~(size_t)15;
     double4* prim = reinterpret_cast<double4*>(reinterpret_cast<char*>(smem_u64) + poff) + wave * 64;
-    for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
+    for (int s = threadIdx.x; s < 2 * P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
     for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
     for (int s = threadIdx.x; s < nreg; s += blockDim.x) {
         np0[s] = 0;
@@ -1272,19 +1453,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
                 cur = k;
                 l1p = lats + k;
                 l2p = l1p;
+                S.gavg = g_avg + (int64_t)k * P.nslots;
+                S.gw = g_w + (int64_t)k * P.nslots;
             }
         }
         // per-cell-pair cost record: wall-clock ticks of the wave on this item
         // (branch-free in the lanes: lane 0 adds dt, the others 0; every lane
         // stores the same compute-unit id)
         const unsigned long long t0 = cp_ticks ? wall_clock64() : 0ull;
-        process_item<MODE, R2F>(P, l1p, l2p, cp, cpp, it, S, qe, prim);
+        process_item<MODE, R2F, GH>(P, l1p, l2p, cp, cpp, it, S, qe, prim);
         if (cp_ticks) {
             const unsigned long long dt = wall_clock64() - t0;
             atomicAdd(&cp_ticks[it.cp], lane == 0 ? dt : 0ull);
             cp_cu[it.cp] = (int)__smid();
         }
     }
+    if (GH) return;
     if (BATCH) {
         if (cur >= 0) {
             __builtin_amdgcn_wave_barrier();
@@ -1296,51 +1480,29 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     flush_region(P, S, R, 0, threadIdx.x, blockDim.x, g_np, g_avg, g_w);
 }
 
-template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_XI, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_XI, R2_X_SUM_YZ, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_XI, R2_FMA_DZ_DY_DX, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_RPPI, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_RPPI, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_WP, RP_SUM, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_WP, RP_FMA, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_SMU, R2_SUM_XY_Z, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_SMU, R2_FMA_DX_DY_DZ, false>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_RPPI_MOCKS, R2_SUM_XY_Z, false>(PairParams, const DevLattice*,
-    const CellPairRec*, const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*,
-    double*, double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_SMU_MOCKS, R2_SUM_XY_Z, false>(PairParams, const DevLattice*,
-    const CellPairRec*, const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*,
-    double*, double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
-template __global__ void k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true>(PairParams, const DevLattice*, const CellPairRec*,
-    const WorkItem*, const int*, int*, const double*, const unsigned short*, unsigned long long*, double*,
-    double*, unsigned long long*, int*, const int*);
+#define CF_TUNED_INST(M, F, B, G)                                                                              \
+    template __global__ void k_pairs_tuned<M, F, B, G>(PairParams, const DevLattice*, const CellPairRec*,      \
+                                                       const WorkItem*, const int*, int*, const double*,       \
+                                                       const unsigned short*, unsigned long long*, double*,    \
+                                                       double*, unsigned long long*, int*, const int*);
+#define CF_TUNED_INST2(M, F) CF_TUNED_INST(M, F, false, false) CF_TUNED_INST(M, F, false, true)
+CF_TUNED_INST2(MODE_DD, R2_SUM_XY_Z)
+CF_TUNED_INST2(MODE_DD, R2_FMA_DZ_DY_DX)
+CF_TUNED_INST2(MODE_XI, R2_SUM_XY_Z)
+CF_TUNED_INST2(MODE_XI, R2_X_SUM_YZ)
+CF_TUNED_INST2(MODE_XI, R2_FMA_DZ_DY_DX)
+CF_TUNED_INST2(MODE_RPPI, RP_SUM)
+CF_TUNED_INST2(MODE_RPPI, RP_FMA)
+CF_TUNED_INST2(MODE_WP, RP_SUM)
+CF_TUNED_INST2(MODE_WP, RP_FMA)
+CF_TUNED_INST2(MODE_SMU, R2_SUM_XY_Z)
+CF_TUNED_INST2(MODE_SMU, R2_FMA_DX_DY_DZ)
+CF_TUNED_INST2(MODE_RPPI_MOCKS, R2_SUM_XY_Z)
+CF_TUNED_INST2(MODE_SMU_MOCKS, R2_SUM_XY_Z)
+CF_TUNED_INST(MODE_DD, R2_SUM_XY_Z, true, false)
+CF_TUNED_INST(MODE_DD, R2_FMA_DZ_DY_DX, true, false)
+#undef CF_TUNED_INST2
+#undef CF_TUNED_INST
 
 // Largest squared separation between the bounding boxes of a cell pair
 // (primary box shifted by the wrap), per the geometry of the mode, and whether

@@ -43,6 +43,8 @@ typedef struct {
     double list_ms;          /* cost prediction, ordering and work-list construction (device events) */
     int hist_copies;         /* LDS histogram replication of the TUNED kernel */
     int autotuned;           /* 0: autotuner off, 1: cached choice used, 2: tuned during this call */
+    int queue_fetch;         /* work items per queue atomic of the TUNED kernel (0 for PLAIN) */
+    int global_hist;         /* 1: the histogram was too large for LDS and was kept in global memory */
 } cf_amd_stats;
 
 /* Autotuner. When on, a call whose options carry the default (non-custom) bin

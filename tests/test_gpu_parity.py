@@ -31,8 +31,10 @@ def _opts(**kw):
 @pytest.mark.parametrize("eng", list(ENGINES))
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
-@pytest.mark.parametrize("isa", ["fallback", "avx512f"])
+@pytest.mark.parametrize("isa", ["fallback", "avx", "avx2", "avx512f"])
 def test_modes_uniform(engine, oracle, eng, mode, dtype, isa):
+    """Every mode under every ISA's FP64 rounding order (xi under AVX/AVX2 sums
+    dx^2 + (dy^2 + dz^2), wp under AVX2 uses an FMA; the rest as fallback or AVX512F)."""
     _select(eng)
     L = 200.0
     X = uniform(20000, L, 11, dtype)
