@@ -85,14 +85,13 @@ struct PairParams {
     // bin lookup: slot count of edges below the 1/16-octave bucket of (float)r^2
     int lut_base;            // bucket index of lut[0]
     int lut_n;               // entries
+    int lut_single;          // 1: no bucket holds more than one edge, so one compare finishes the lookup
     int dense_mode;          // 0: by work-item flag, 1: always dense, 2: never dense
     int hist_copies;         // LDS histogram replication R (power of 2): lane l adds to copy l % R
     int dense_stride;        // 1: dense work items take strided primaries
     float dense_frac;        // a cell pair whose estimated in-range fraction reaches this takes the dense path
     int fetch;               // work items per queue atomic outside the single-fetch ranges
     int prim_cache;          // 1: the sparse path keeps the wave's primaries in LDS for its drains
-    int fix_move_bits;       // fixed-point words of magnitude >= 2^fix_move_bits move to global memory (61)
-    int fix_check_calls;     // histogram updates between overflow checks, per workgroup (2^13)
     int max_window;          // longest secondary window scanned with one queue base (2^25, a multiple of 4)
 };
 

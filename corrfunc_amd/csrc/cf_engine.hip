@@ -668,6 +668,25 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         P.lut_base = base;
         P.lut_n = (int)lut.size();
         if (nbin > 65535) P.sorted_edges = 0;
+        // The kernels step up from lut[idx] while e[k] <= r2. A pair in bucket idx
+        // has thr(idx) <= r2 < lo(idx + 1) * (1 + 2^-24) ((float)r2 rounds by at
+        // most 2^-24), so at most one step is needed when no such interval holds
+        // two edges e[0..nbin-2].
+        P.lut_single = 1;
+        for (int idx = 0; idx < n && P.lut_single; idx++) {
+            auto lo_of = [&](int b) {
+                const uint32_t bits = (uint32_t)(base + b) << 19;
+                float f;
+                memcpy(&f, &bits, 4);
+                return (double)f;
+            };
+            const double thr = lo_of(idx) * (1.0 - 0x1p-20);
+            const double top = idx + 1 < n ? lo_of(idx + 1) * (1.0 + 0x1p-23) : INFINITY;
+            int cnt = 0;
+            for (int m = 0; m + 1 < nbin; m++)
+                if (rupp_sqr[m] >= thr && rupp_sqr[m] < top) cnt++;
+            if (cnt > 1) P.lut_single = 0;
+        }
     }
     {
         const char* dm = getenv("CORRFUNC_AMD_DENSE");
@@ -675,13 +694,10 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         const char* ds = getenv("CORRFUNC_AMD_STRIDE");
         P.dense_stride = ds ? atoi(ds) : 1;
         const char* df = getenv("CORRFUNC_AMD_DENSE_FRAC");
-        P.dense_frac = df ? (float)atof(df) : 0.35f;
-        // overflow moves of the fixed-point histogram sums (k_pairs_tuned): the
-        // variables exist to exercise the move path in tests
-        const char* fm = getenv("CORRFUNC_AMD_FIX_MOVE_BITS");
-        P.fix_move_bits = fm ? std::max(1, std::min(61, atoi(fm))) : 61;
-        const char* fc = getenv("CORRFUNC_AMD_FIX_CHECK_CALLS");
-        P.fix_check_calls = fc ? std::max(1, std::min(1 << 13, atoi(fc))) : 1 << 13;
+        // measured (bench.py --sweep CORRFUNC_AMD_DENSE_FRAC): DDsmu cross
+        // (workload 5) 2.71 s at 0.35, 2.60 s at 0.2-0.25, 3.18 s at 0; DDrppi
+        // and DD within 2 % from 0.2 to 0.5
+        P.dense_frac = df ? (float)atof(df) : 0.25f;
         // sub-window length of the sparse scan (tests shorten it to exercise the
         // window changes; a multiple of 4 below 2^26)
         const char* mw = getenv("CORRFUNC_AMD_MAX_WINDOW");
@@ -691,46 +707,6 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         if (eng().force_fetch > 0) P.fetch = eng().force_fetch;
     }
 
-}
-
-// What the pair kernels read as `rupp_sqr`: the squared edges [nbin], the
-// power-of-2 scales of the fixed-point separation sums per bin [nbin] and the
-// weight-product scale [1] (written on the device by weight_scale_device; 0
-// here). The scale of bin k makes the largest separation a pair of bin k can
-// have -- the largest edge at or above k, rmax for sorted edges -- at most
-// 2^42 fixed-point units.
-static std::vector<double> edge_table(const std::vector<double>& rupp, const std::vector<double>& rupp_sqr, int nbin)
-{
-    std::vector<double> h(2 * (size_t)nbin + 1, 0.0);
-    double hi = 0.0;
-    for (int k = nbin - 1; k >= 0; k--) {
-        h[k] = rupp_sqr[k];
-        if (std::isfinite(rupp[k])) hi = fmax(hi, rupp[k]);
-        double sc = 1.0;
-        if (hi > 0.0) {
-            int ex = 0;
-            frexp(hi, &ex);
-            sc = ldexp(1.0, std::max(-1000, std::min(1000, 42 - ex)));
-        }
-        h[nbin + k] = sc;
-    }
-    return h;
-}
-
-// Scale of the fixed-point weight-product sums from max|w| of the sets
-// (k_weight_scale), written to *d_wsc on the stream.
-static void weight_scale_device(const DevLattice* sets, int nsets, bool same, double* d_wsc)
-{
-    Engine& E = eng();
-    unsigned long long* m = (unsigned long long*)E.arena.get("wmax", 2 * sizeof(unsigned long long));
-    k_fill_u64<<<1, 64, 0, E.stream>>>(2, m, 0ull);
-    for (int k = 0; k < nsets; k++) {
-        const DevLattice& L = sets[k];
-        unsigned long long* mk = (same || k == 0) ? m : m + 1;
-        if (L.n > 0 && L.w) k_absmax<<<(unsigned)std::min<int64_t>(1024, (L.n + 255) / 256), 256, 0, E.stream>>>(L.n, L.w, mk);
-    }
-    k_weight_scale<<<1, 64, 0, E.stream>>>(m, same ? m : m + 1, d_wsc);
-    CF_CHECK(hipGetLastError());
 }
 
 // Device copy of the (primary, secondary) lattice pair of a single call.
@@ -806,9 +782,9 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     Engine& E = eng();
     const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
     // LDS of everything but the histogram for a workgroup of `nw` waves:
-    // squared edges and sum scales, bin lookup, survivor queues
+    // squared edges, bin lookup, survivor queues
     auto lds_fixed_for = [&](int nw) {
-        return (size_t)nbin * 16 + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nw * kQueueAlloc * 4;
+        return (size_t)nbin * 8 + (size_t)((P.lut_n + 1) & ~1) * 2 + (size_t)nw * kQueueAlloc * 4;
     };
     // a histogram whose single copy does not fit next to the rest of a 4-wave
     // workgroup is kept in global memory (no replication; FP64 atomics)
@@ -1353,16 +1329,11 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     unsigned long long* g_np = (unsigned long long*)E.arena.get("h_np", sizeof(uint64_t) * nslots);
     double* g_avg = (double*)E.arena.get("h_avg", sizeof(double) * nslots);
     double* g_w = (double*)E.arena.get("h_w", sizeof(double) * nslots);
-    const std::vector<double> etab = edge_table(R.rupp, rupp_sqr, nbin);
-    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * etab.size());
+    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * nbin);
     k_fill_u64<<<(unsigned)((nslots + 255) / 256), 256, 0, E.stream>>>(nslots, g_np, 0ull);
     k_fill_u64<<<(unsigned)((nslots + 255) / 256), 256, 0, E.stream>>>(nslots, (unsigned long long*)g_avg, 0ull);
     k_fill_u64<<<(unsigned)((nslots + 255) / 256), 256, 0, E.stream>>>(nslots, (unsigned long long*)g_w, 0ull);
-    CF_CHECK(hipMemcpyAsync(d_edges, etab.data(), sizeof(double) * etab.size(), hipMemcpyHostToDevice, E.stream));
-    if (need_w) {
-        const DevLattice sets[2] = {L1, L2};
-        weight_scale_device(sets, c.autocorr ? 1 : 2, c.autocorr != 0, d_edges + 2 * nbin);
-    }
+    CF_CHECK(hipMemcpyAsync(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice, E.stream));
     unsigned short* d_lut = (unsigned short*)E.arena.get("lut", sizeof(unsigned short) * lut.size());
     CF_CHECK(hipMemcpyAsync(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice, E.stream));
 
@@ -1631,15 +1602,13 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     unsigned long long* g_np = (unsigned long long*)E.arena.get("bh_np", sizeof(uint64_t) * tot);
     double* g_avg = (double*)E.arena.get("bh_avg", sizeof(double) * tot);
     double* g_w = (double*)E.arena.get("bh_w", sizeof(double) * tot);
-    const std::vector<double> etab = edge_table(rupp, rupp_sqr, nbin);
-    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * etab.size());
+    double* d_edges = (double*)E.arena.get("edges", sizeof(double) * nbin);
     unsigned short* d_lut = (unsigned short*)E.arena.get("lut", sizeof(unsigned short) * lut.size());
     const unsigned fb = (unsigned)std::min<size_t>(4096, (tot + 255) / 256);
     k_fill_u64<<<fb, 256, 0, E.stream>>>((int64_t)tot, g_np, 0ull);
     k_fill_u64<<<fb, 256, 0, E.stream>>>((int64_t)tot, (unsigned long long*)g_avg, 0ull);
     k_fill_u64<<<fb, 256, 0, E.stream>>>((int64_t)tot, (unsigned long long*)g_w, 0ull);
-    CF_CHECK(hipMemcpyAsync(d_edges, etab.data(), sizeof(double) * etab.size(), hipMemcpyHostToDevice, E.stream));
-    if (need_w) weight_scale_device(hl.data(), ncat, true, d_edges + 2 * nbin);
+    CF_CHECK(hipMemcpyAsync(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice, E.stream));
     CF_CHECK(hipMemcpyAsync(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice, E.stream));
 
     int nitems = 0;

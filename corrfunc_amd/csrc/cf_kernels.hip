@@ -414,43 +414,6 @@ __global__ void k_cellpair_emit(NgbArgs a, int64_t ncells, int nngb, const int* 
     }
 }
 
-// ------------------------------------------- scales of the fixed-point sums
-// max |w| of a weight array as the bit pattern of a non-negative double
-// (non-negative doubles order like their bits; NaN orders above +inf), max-ed
-// into *out (zeroed by the caller).
-__global__ void k_absmax(int64_t n, const double* __restrict__ w, unsigned long long* __restrict__ out)
-{
-    unsigned long long m = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(w[i]));
-        m = b > m ? b : m;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long t = __shfl_xor(m, o, 64);
-        m = t > m ? t : m;
-    }
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
-}
-
-// Power-of-2 scale of the weight-product sums: max|w1| * max|w2| * scale <= 2^41.
-// 0 (sums in FP64 in global memory) when a weight is not finite or the
-// products are too small for a normal scale; 1 when every weight is 0.
-__global__ void k_weight_scale(const unsigned long long* __restrict__ m1, const unsigned long long* __restrict__ m2,
-                               double* __restrict__ out)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double m = __longlong_as_double((long long)*m1) * __longlong_as_double((long long)*m2);
-    double s = 0.0;
-    if (m == 0.0) {
-        s = 1.0;
-    } else if (isfinite(m)) {
-        int ex = 0;
-        frexp(m, &ex);
-        if (41 - ex < 1000) s = ldexp(1.0, 41 - ex);
-    }
-    *out = s;
-}
-
 // ------------------------------------------------- exact pair replay (a8-a12)
 __device__ __forceinline__ double r2_3d(int f, double dx, double dy, double dz)
 {
@@ -768,104 +731,26 @@ __device__ __forceinline__ double fast_sqrt(double r2)
 
 // Histogram of one workgroup of the tuned kernel, replicated R ways with layout
 // [slot][R]: lanes of one wave that land in the same slot add to different
-// copies, which sit in different banks. Counts are u64; the sums of
-// separations and of weight products are exact fixed-point integers (fix_add),
-// so every update is a 64-bit integer LDS atomic (the FP64 LDS atomic costs
-// about 20x more per update on gfx950). GH kernels (histograms larger than
-// the workgroup's LDS) add to the global FP64 histogram instead.
+// copies, which sit in different banks. Counts u64, sums FP64 (native
+// ds_add_u64 / ds_add_f64: the LDS atomics issue on the LDS pipe, not on the
+// VALU, which bounds this kernel). GH kernels (histograms larger than the
+// workgroup's LDS) add to the global histogram instead.
 struct KernelLds {
     unsigned long long* np;
-    long long* avg;            // fixed point, scale sc[bin]
-    long long* w;              // fixed point, scale wsc
+    double* avg;
+    double* w;
     int rmask;                 // R - 1
     int lane_copy;             // lane & (R - 1)
     const double* e;           // rupp_sqr[0..nbin-1]
-    const double* sc;          // [nbin] power-of-2 scale of the separation sums per bin
     const unsigned short* lut; // bin lookup
-    double wsc;                // power-of-2 scale of the weight-product sums; 0: FP64 in global memory
-    unsigned long long* gnp;   // the global histogram this region flushes to
+    unsigned long long* gnp;   // the global histogram (GH)
     double* gavg;
     double* gw;
-    int calls;                 // histogram updates (of <= 64 lanes) since the last overflow check
-    int calls_lim;
 };
 
-// Fixed-point sums. An addend x becomes q = round(x * scale) with scale a power
-// of 2 chosen so that |q| <= 2^43 (sc[k] * rupp[k] <= 2^42 for the separations of
-// bin k; wsc * max|w1| * max|w2| <= 2^41). One FP64 FMA with 1.5 * 2^52 puts
-// round(x * scale) in the low mantissa bits (the sum lies in [2^52, 2^53), where
-// the spacing of doubles is 1). Addends with |q| < 2^37 (x below 1/16-1/32 of the
-// bin's upper edge, or tiny weights) are added in FP64 to the global histogram
-// instead, so every fixed-point addend has a relative rounding error below
-// 0.5 / 2^37 = 3.6e-12 and the sums agree with FP64 sums to that bound.
-// Overflow: a wave checks its workgroup's region every calls_lim updates (of
-// <= 64 addends each, calls_lim = 2^13 / waves) and moves every word with
-// |value| >= 2^61 to the global histogram; between two checks of a word by any
-// wave, the waves add less than 2^19 * 2^43 = 2^62 to it, so no word leaves int64.
-// (The move threshold 2^61 and the check interval 2^13 are PairParams fields,
-// fix_move_bits and fix_check_calls, so that tests can force frequent moves.)
-constexpr long long kFixEscape = 1LL << 37;
-constexpr long long kFixMagicBits = 0x4338000000000000LL;  // bits of 1.5 * 2^52
-
-__device__ __forceinline__ long long fix_q(double x, double scale)
-{
-    return __double_as_longlong(fma(x, scale, 0x1.8p52)) - kFixMagicBits;
-}
-
-__device__ __forceinline__ void fix_add(long long* p, double* gp, double x, double scale)
-{
-    const long long q = fix_q(x, scale);
-    if (q < kFixEscape && q > -kFixEscape && x != 0.0)
-        atomicAdd(gp, x);
-    else
-        atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)q);
-}
-
-// histogram bin (rp / s / r bin) of a slot
-__device__ __forceinline__ int slot_bin(const PairParams& P, int slot) { return P.n2 > 0 ? slot / (P.n2 + 1) : slot; }
-
-// Moves every fixed-point word of the wave's region whose magnitude reached
-// kFixMove to the global histogram (one wave, all lanes; other waves may be
-// adding to the same words: the move is an atomic exchange with 0).
-__device__ void fix_check(const PairParams& P, KernelLds& S)
-{
-    const int R = S.rmask + 1;
-    const int n = P.nslots * R;
-    const long long kFixMove = 1LL << P.fix_move_bits;
-    for (int t = lane_id(); t < n; t += 64) {
-        if (P.need_avg) {
-            const long long v = __hip_atomic_load(&S.avg[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (v >= kFixMove || v <= -kFixMove) {
-                const long long o = (long long)atomicExch(reinterpret_cast<unsigned long long*>(&S.avg[t]), 0ull);
-                atomicAdd(&S.gavg[t / R], (double)o / S.sc[slot_bin(P, t / R)]);
-            }
-        }
-        if (P.need_w && S.wsc != 0.0) {
-            const long long v = __hip_atomic_load(&S.w[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (v >= kFixMove || v <= -kFixMove) {
-                const long long o = (long long)atomicExch(reinterpret_cast<unsigned long long*>(&S.w[t]), 0ull);
-                atomicAdd(&S.gw[t / R], (double)o / S.wsc);
-            }
-        }
-    }
-}
-
-// Counts one histogram update of the wave (wave-uniform control flow) and runs
-// the overflow check when due.
+// Adds a counted pair (slot, squared separation r2, weight product) to the histogram.
 template <bool GH>
-__device__ __forceinline__ void fix_tick(const PairParams& P, KernelLds& S)
-{
-    if (GH) return;
-    if (++S.calls >= S.calls_lim) {
-        S.calls = 0;
-        fix_check(P, S);
-    }
-}
-
-// Adds a counted pair (slot, bin k, squared separation r2, weight product) to
-// the histogram.
-template <bool GH>
-__device__ __forceinline__ void hist_pair(const PairParams& P, KernelLds& S, int slot, int k, double r2, double wprod)
+__device__ __forceinline__ void hist_pair(const PairParams& P, KernelLds& S, int slot, double r2, double wprod)
 {
     if (GH) {
         atomicAdd(&S.gnp[slot], 1ull);
@@ -875,13 +760,8 @@ __device__ __forceinline__ void hist_pair(const PairParams& P, KernelLds& S, int
     }
     const int a = slot * (S.rmask + 1) + S.lane_copy;
     atomicAdd(&S.np[a], 1ull);
-    if (P.need_avg) fix_add(&S.avg[a], &S.gavg[slot], fast_sqrt(r2), S.sc[k]);
-    if (P.need_w) {
-        if (S.wsc != 0.0)
-            fix_add(&S.w[a], &S.gw[slot], wprod, S.wsc);
-        else
-            atomicAdd(&S.gw[slot], wprod);
-    }
+    if (P.need_avg) atomicAdd(&S.avg[a], fast_sqrt(r2));
+    if (P.need_w) atomicAdd(&S.w[a], wprod);
 }
 
 // Number of edges e[0..nbin-2] <= r2 (the reference's top-down bin search,
@@ -893,6 +773,7 @@ __device__ __forceinline__ int lut_kbin(const PairParams& P, const KernelLds& S,
     int idx = (int)(__float_as_uint((float)r2) >> 19) - P.lut_base;
     idx = idx < 0 ? 0 : (idx >= P.lut_n ? P.lut_n - 1 : idx);
     int k = S.lut[idx];
+    if (P.lut_single) return k + (S.e[k] <= r2 ? 1 : 0);  // e[nbin-1] = rmax^2 > r2
     while (k < P.nbin - 1 && S.e[k] <= r2) k++;
     return k;
 }
@@ -952,7 +833,7 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
         }
         const int k = lut_kbin(P, S, key);
         if (k == 0) return;
-        hist_pair<GH>(P, S, k * (P.n2 + 1) + j2, k, key, wi * wj);
+        hist_pair<GH>(P, S, k * (P.n2 + 1) + j2, key, wi * wj);
         return;
     }
     const double sqr_dz = dz * dz;
@@ -983,7 +864,10 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
         int idx = (int)(__float_as_uint((float)r2) >> 19) - P.lut_base;
         idx = idx < 0 ? 0 : (idx >= P.lut_n ? P.lut_n - 1 : idx);
         k = S.lut[idx];
-        if (k < P.nbin - 1 && S.e[k] <= r2) {
+        if (P.lut_single) {
+            // at most one edge above lut[idx] can lie below r2 (e[nbin-1] = rmax^2 > r2)
+            k += S.e[k] <= r2 ? 1 : 0;
+        } else if (k < P.nbin - 1 && S.e[k] <= r2) {
             k++;
             while (k < P.nbin - 1 && S.e[k] <= r2) k++;
         }
@@ -1001,16 +885,31 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
             slot = k * (P.n2 + 1) + pb;
         }
     } else if (MODE == MODE_SMU) {
-        const double mu = sqrt(sqr_dz / r2);
-        if (P.float_index) {
-            slot = (int)((double)k * (double)(P.n2 + 1) + mu * P.inv_d2);
+        // mu bin from an FP32 estimate t ~ mu * inv_dmu: |dz|, s^2 and inv_dmu
+        // rounded to FP32 (2^-24 each), the hardware reciprocal square root (1
+        // ulp) and two products put t within 4e-7 * t of the reference's FP64
+        // value, so its integer part is the reference's unless t lies within
+        // 1e-6 * (t + 1) of an integer; only those pairs (and non-finite
+        // estimates, clamped to 1e9) compute mu = sqrt(dz^2 / s^2) in FP64 as the
+        // reference does (countpairs_s_mu_kernels.c.src:1154-1171).
+        const float t = fminf((float)adz * __builtin_amdgcn_rsqf((float)r2) * (float)P.inv_d2, 1e9f);
+        const int mbf = (int)t;
+        const float fr = t - (float)mbf;
+        const float tol = 1e-6f * (t + 1.0f);
+        if (fr >= tol && fr <= 1.0f - tol) {
+            slot = k * (P.n2 + 1) + (mbf > P.n2 ? P.n2 : mbf);
         } else {
-            int mb = (int)(mu * P.inv_d2);
-            mb = mb > P.n2 ? P.n2 : mb;
-            slot = k * (P.n2 + 1) + mb;
+            const double mu = sqrt(sqr_dz / r2);
+            if (P.float_index) {
+                slot = (int)((double)k * (double)(P.n2 + 1) + mu * P.inv_d2);
+            } else {
+                int mb = (int)(mu * P.inv_d2);
+                mb = mb > P.n2 ? P.n2 : mb;
+                slot = k * (P.n2 + 1) + mb;
+            }
         }
     }
-    hist_pair<GH>(P, S, slot, k, r2, wi * wj);
+    hist_pair<GH>(P, S, slot, r2, wi * wj);
 }
 
 // Decides up to 64 queued survivors (entries ((j - jbase) << 6) | lane) in FP64.
@@ -1038,7 +937,6 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
         const double4 pj = d4_2[j];
         exact_pair<MODE, R2F, GH>(P, cpp, l1p, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
-    fix_tick<GH>(P, S);
 }
 
 // Integer pair filter on quantized cell-local positions (secondary s, primary
@@ -1231,7 +1129,6 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
             const double4 nxt = kload_d4(ds, j + 1);
             exact_pair<MODE, R2F, GH>(P, cpp, l1p, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
                                       P.need_w ? cur.w : 0.0, S);
-            fix_tick<GH>(P, S);
             cur = nxt;
         }
         return;
@@ -1310,9 +1207,7 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
 }
 
 // Adds a (replicated) LDS histogram region to the global histogram at `base`
-// and clears it; `t0`/`tstep` split the slots among the calling threads. The
-// fixed-point sums of the R copies are converted to FP64 (exact integers below
-// 2^63, scaled by a power of 2) and summed in FP64.
+// and clears it; `t0`/`tstep` split the slots among the calling threads.
 __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, int R, int64_t base, int t0,
                                              int tstep, unsigned long long* g_np, double* g_avg, double* g_w)
 {
@@ -1322,13 +1217,13 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
         for (int r = 0; r < R; r++) {
             c += S.np[sl * R + r];
             S.np[sl * R + r] = 0;
-            if (P.need_avg) { sa += (double)S.avg[sl * R + r]; S.avg[sl * R + r] = 0; }
-            if (P.need_w) { sw += (double)S.w[sl * R + r]; S.w[sl * R + r] = 0; }
+            if (P.need_avg) { sa += S.avg[sl * R + r]; S.avg[sl * R + r] = 0; }
+            if (P.need_w) { sw += S.w[sl * R + r]; S.w[sl * R + r] = 0; }
         }
         if (c) {
             atomicAdd(&g_np[base + sl], c);
-            if (P.need_avg) atomicAdd(&g_avg[base + sl], sa / S.sc[slot_bin(P, sl)]);
-            if (P.need_w && S.wsc != 0.0) atomicAdd(&g_w[base + sl], sw / S.wsc);
+            if (P.need_avg) atomicAdd(&g_avg[base + sl], sa);
+            if (P.need_w) atomicAdd(&g_w[base + sl], sw);
         }
     }
 }
@@ -1338,9 +1233,7 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
 // `pad`, both sets are lats[k], each wave keeps its own LDS histogram for the
 // catalog it is working on, and the histogram of catalog k lives at
 // g_*[k*nslots ...]. GH = true: no LDS histogram; every counted pair is added
-// to the global FP64 histogram (histograms too large for the workgroup's LDS).
-// rupp_sqr holds the squared edges [nbin], the per-bin scales of the
-// separation sums [nbin] and the weight-product scale [1] (fix_add).
+// to the global histogram (histograms too large for the workgroup's LDS).
 template <int MODE, int R2F, bool BATCH, bool GH>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
@@ -1361,26 +1254,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int nreg = BATCH ? nrep * nwaves : nrep;  // all regions of the workgroup
     const int wave = threadIdx.x >> 6, lane = lane_id();
     unsigned long long* np0 = smem_u64;
-    long long* avg0 = reinterpret_cast<long long*>(smem_u64 + nreg);
-    long long* w0 = avg0 + (P.need_avg ? nreg : 0);
-    double* e = reinterpret_cast<double*>(w0 + (P.need_w ? nreg : 0));
-    double* sc = e + P.nbin;
+    double* avg0 = reinterpret_cast<double*>(smem_u64 + nreg);
+    double* w0 = avg0 + (P.need_avg ? nreg : 0);
+    double* e = w0 + (P.need_w ? nreg : 0);
     const int roff = BATCH ? wave * nrep : 0;
     S.np = np0 + roff;
     S.avg = avg0 + roff;
     S.w = w0 + roff;
     S.rmask = R - 1;
     S.lane_copy = lane & (R - 1);
-    S.wsc = rupp_sqr[2 * P.nbin];
     S.gnp = g_np;
     S.gavg = g_avg;
     S.gw = g_w;
-    S.calls = 0;
-    S.calls_lim = max(1, P.fix_check_calls / (BATCH ? 1 : nwaves));
-    unsigned short* lut = reinterpret_cast<unsigned short*>(sc + P.nbin);
+    unsigned short* lut = reinterpret_cast<unsigned short*>(e + P.nbin);
     int* qbase = reinterpret_cast<int*>(lut + ((P.lut_n + 1) & ~1));
     S.e = e;
-    S.sc = sc;
     S.lut = lut;
     unsigned* qe = reinterpret_cast<unsigned*>(qbase) + __builtin_amdgcn_readfirstlane(wave) * kQueueAlloc;
     // primaries cache: 16-byte aligned after the queues
@@ -1388,7 +1276,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const size_t poff = ((size_t)(reinterpret_cast<char*>(qbase + nwaves * kQueueAlloc) -
                                   reinterpret_cast<char*>(smem_u64)) + 15) & ~(size_t)15;
     double4* prim = reinterpret_cast<double4*>(reinterpret_cast<char*>(smem_u64) + poff) + wave * 64;
-    for (int s = threadIdx.x; s < 2 * P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
+    for (int s = threadIdx.x; s < P.nbin; s += blockDim.x) e[s] = rupp_sqr[s];
     for (int s = threadIdx.x; s < P.lut_n; s += blockDim.x) lut[s] = lut_g[s];
     for (int s = threadIdx.x; s < nreg; s += blockDim.x) {
         np0[s] = 0;
@@ -1453,8 +1341,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
                 cur = k;
                 l1p = lats + k;
                 l2p = l1p;
-                S.gavg = g_avg + (int64_t)k * P.nslots;
-                S.gw = g_w + (int64_t)k * P.nslots;
             }
         }
         // per-cell-pair cost record: wall-clock ticks of the wave on this item

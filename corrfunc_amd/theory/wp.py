@@ -30,7 +30,8 @@ def find_fastest_wp_bin_refs(boxsize, pimax, nthreads, binfile, X, Y, Z,
     dtype = np.dtype([("nx", np.int64), ("ny", np.int64), ("nz", np.int64), ("avg_time", np.float64),
                       ("sigma_time", np.float64)])
     all_runtimes = np.zeros(maxbinref ** 3, dtype=dtype)
-    all_runtimes[:] = np.inf
+    all_runtimes["avg_time"] = np.inf
+    all_runtimes["sigma_time"] = np.inf
     for ii, (nx, ny, nz) in enumerate(itertools.product(bin_refs, bin_refs, bin_refs)):
         total = total_sqr = 0.0
         failed = False

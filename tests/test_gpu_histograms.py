@@ -3,9 +3,9 @@
 * histograms of any size: (nbin+1)*(n2+1) slots as the reference allocates
   (theory/DDrppi/countpairs_rp_pi_impl.c.src:207, theory/DDsmu/countpairs_s_mu_impl.c.src:226),
   in LDS where one copy fits the workgroup, else in global memory;
-* the fixed-point LDS sums of separations and weight products: wide-range,
-  negative and zero weights, separations far below their bin's upper edge,
-  forced overflow moves, and the windowed survivor queue;
+* the sums of separations and weight products: wide-range, negative and zero
+  weights, separations far below their bin's upper edge, coincident points;
+* the survivor queue's window-relative indices (windows forced short);
 * wp's c_api_time in nanoseconds (theory/wp/countpairs_wp_impl.c.src:669-673);
 * concurrent calls from two host threads.
 
@@ -48,9 +48,11 @@ LARGE = [
     ("countpairs_s_mu", np.linspace(0.0, 30.0, 101), 100, "global"),
     # 3,001 edges of a 1-D histogram (70 KiB: one LDS copy, the plain kernel's global)
     ("countpairs", np.logspace(-1, np.log10(30.0), 3001), None, "lds"),
-    # 10,001 edges: the edges and their scales (16 B each) exceed a workgroup's
-    # LDS next to the queues: the call falls back to the plain kernel
-    ("countpairs_xi", np.logspace(-1, np.log10(30.0), 10001), None, "plain"),
+    # 10,001 edges of a 1-D histogram: 235 KiB of slots, global memory
+    ("countpairs_xi", np.logspace(-1, np.log10(30.0), 10001), None, "global"),
+    # 20,001 edges: the edges alone (8 B each) exceed a workgroup's LDS next to
+    # the queues: the call falls back to the plain kernel
+    ("countpairs", np.logspace(-1, np.log10(30.0), 20001), None, "plain"),
 ]
 
 
@@ -59,9 +61,14 @@ LARGE = [
 def test_large_histograms(engine, oracle, select, eng, case):
     mode, bins, extra, where = case
     select(eng)
-    L = 200.0
-    X = clustered(30000, L, 41, nparent=30, sigma=3.0, dtype=np.float64)
-    W = weights(30000, 42, np.float64)
+    # pimax = 100 must stay below half the box; a smaller uniform catalog where
+    # the oracle's top-down search walks 20,000 edges
+    L = 250.0
+    if where == "plain":
+        X = uniform(6000, L, 43)
+    else:
+        X = clustered(40000, L, 41, nparent=30, sigma=3.0, dtype=np.float64)
+    W = weights(len(X[0]), 42, np.float64)
     kw = mode_kwargs(mode, X, W, autocorr=1, bins=bins, L=L,
                      pimax=extra if mode == "countpairs_rp_pi" else None,
                      mu_max=1.0, nmu_bins=extra if mode == "countpairs_s_mu" else 10)
@@ -83,10 +90,8 @@ def test_large_histograms(engine, oracle, select, eng, case):
 @pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_s_mu"])
 @pytest.mark.parametrize("wkind", ["wide", "signed", "zero", "outlier"])
 def test_fixed_point_weight_sums(engine, oracle, select, eng, mode, wkind):
-    """Weight products of any range and sign. Products below 2^-4..2^-5 of the
-    largest go to the FP64 global sums; the rest are fixed point. With signed
-    weights a bin's sum can cancel, so it is compared against the bound the
-    fixed-point rounding gives: 1e-10 of the sum of |w_i w_j|."""
+    """Weight products of any range and sign. With signed weights a bin's sum
+    can cancel, so it is compared against 1e-10 of the sum of |w_i w_j|."""
     select(eng)
     L = 150.0
     n = 20000
@@ -119,8 +124,8 @@ def test_fixed_point_weight_sums(engine, oracle, select, eng, mode, wkind):
 
 @pytest.mark.parametrize("mode", ["countpairs", "countpairs_s_mu"])
 def test_separations_far_below_their_bin_edge(engine, oracle, select, mode):
-    """A first bin [0, 5) holding pairs at separations of 1e-7 .. 1e-3 (the
-    FP64 global path) next to ordinary ones (fixed point), and coincident points."""
+    """A first bin [0, 5) holding pairs at separations of 1e-7 .. 1e-3 next to
+    ordinary ones, and coincident points."""
     select("tuned")
     L = 100.0
     rng = np.random.default_rng(61)
@@ -137,15 +142,14 @@ def test_separations_far_below_their_bin_edge(engine, oracle, select, mode):
     assert_same(mode, got, want)
 
 
-@pytest.mark.parametrize("knobs", [{"CORRFUNC_AMD_FIX_MOVE_BITS": "20", "CORRFUNC_AMD_FIX_CHECK_CALLS": "1"},
-                                   {"CORRFUNC_AMD_FIX_MOVE_BITS": "44", "CORRFUNC_AMD_FIX_CHECK_CALLS": "8"},
-                                   {"CORRFUNC_AMD_MAX_WINDOW": "8"},
-                                   {"CORRFUNC_AMD_MAX_WINDOW": "100", "CORRFUNC_AMD_DENSE": "2"}],
-                         ids=["move_always", "move_often", "window_8", "window_100_sparse"])
-@pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_wp"])
-def test_forced_moves_and_windows(engine, oracle, select, monkeypatch, knobs, mode):
-    """Overflow moves of fixed-point words to global memory (forced by a low
-    threshold) and survivor-queue windows (forced short) keep the counts and sums."""
+@pytest.mark.parametrize("knobs", [{"CORRFUNC_AMD_MAX_WINDOW": "8"},
+                                   {"CORRFUNC_AMD_MAX_WINDOW": "100", "CORRFUNC_AMD_DENSE": "2"},
+                                   {"CORRFUNC_AMD_MAX_WINDOW": "4096", "CORRFUNC_AMD_DENSE": "2"}],
+                         ids=["window_8", "window_100_sparse", "window_4096_sparse"])
+@pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_wp", "countpairs_s_mu"])
+def test_short_windows(engine, oracle, select, monkeypatch, knobs, mode):
+    """Secondary windows scanned in short sub-windows (each with its own queue
+    base, drained at its end) keep the counts and sums."""
     select("tuned")
     for k, v in knobs.items():
         monkeypatch.setenv(k, v)
