@@ -50,12 +50,44 @@ struct InputError {};
 
 // --------------------------------------------------------------- state
 // Grow-only device buffers, reused across calls (the entry points are not
-// re-entrant, like the reference's).
+// re-entrant, like the reference's). In fixed mode (cf_amd_countpairs_noalloc)
+// every buffer of the call is carved out of one caller-independent device
+// region instead, 256-byte aligned, and nothing is allocated or freed; a call
+// that needs more than the region throws InputError after printing why.
 struct Arena {
     std::unordered_map<std::string, std::pair<void*, size_t>> slots;
+    char* fixed = nullptr;   // base of the region in fixed mode, else nullptr
+    size_t fixed_cap = 0, fixed_off = 0;
+    std::unordered_map<std::string, std::pair<void*, size_t>> fixed_slots;
+    void begin_fixed(void* base, size_t cap)
+    {
+        fixed = (char*)base;
+        fixed_cap = cap;
+        fixed_off = 0;
+        fixed_slots.clear();
+    }
+    void end_fixed()
+    {
+        fixed = nullptr;
+        fixed_slots.clear();
+    }
     void* get(const std::string& name, size_t bytes)
     {
         if (bytes == 0) bytes = 256;
+        if (fixed) {
+            auto f = fixed_slots.find(name);
+            if (f != fixed_slots.end() && f->second.second >= bytes) return f->second.first;
+            const size_t off = (fixed_off + 255) & ~(size_t)255;
+            if (off > fixed_cap || bytes > fixed_cap - off) {
+                fprintf(stderr, "Error: the call needs more than the %zu bytes of the engine's fixed device "
+                                "scratch region (at %s: %zu bytes more); use the allocating entry point for "
+                                "inputs this large\n", fixed_cap, name.c_str(), bytes);
+                throw InputError();
+            }
+            fixed_off = off + bytes;
+            fixed_slots[name] = {fixed + off, bytes};
+            return fixed + off;
+        }
         auto it = slots.find(name);
         if (it != slots.end() && it->second.second >= bytes) return it->second.first;
         if (it != slots.end()) {
@@ -85,9 +117,19 @@ struct CostModel {
     int* d_count = nullptr;     // samples appended so far (may exceed cap)
 };
 
+// Device scratch of cf_amd_countpairs_noalloc: every buffer that entry point's
+// kernels use besides its inputs and outputs, carved from this array (the
+// engine's fixed arena mode). Each GPU the library runs on holds its own copy
+// for the life of the process. Host code never passes the array's name to a
+// kernel or a copy: it is a host-side stand-in; the device address comes from
+// hipGetSymbolAddress (Engine::scratch_ptr).
+constexpr size_t kScratchBytes = CF_AMD_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct Engine {
     bool init = false;
     int device = 0;
+    void* scratch_ptr = nullptr;  // device address of `scratch` on `device` (nullptr: lookup failed)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_list = nullptr;
     Arena arena;
@@ -137,6 +179,16 @@ static Engine& eng()
             E.ns_per_tick = 1e6 / khz;
         if (const char* at = getenv("CORRFUNC_AMD_AUTOTUNE")) E.autotune = atoi(at) != 0;
         memset(&E.last, 0, sizeof E.last);
+        // the scratch array's address on this device, once, with the device current
+        void* sp = nullptr;
+        const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+        if (se == hipSuccess) {
+            E.scratch_ptr = sp;
+        } else {
+            (void)hipGetLastError();
+            fprintf(stderr, "corrfunc_amd: hipGetSymbolAddress(scratch) failed: %s; cf_amd_countpairs_noalloc is "
+                            "unavailable\n", hipGetErrorString(se));
+        }
         E.init = true;
     }
     return E;
@@ -147,6 +199,24 @@ static double now_s()
     struct timeval t;
     gettimeofday(&t, NULL);
     return t.tv_sec + 1e-6 * t.tv_usec;
+}
+
+// Clears device words with a fill kernel on the engine's stream (no memset
+// nodes, so the sequence can be captured into a graph).
+static void fill_i32(int* p, int64_t n, int v)
+{
+    Engine& E = eng();
+    if (n <= 0) return;
+    k_fill_i32<<<(unsigned)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, E.stream>>>(n, p, v);
+    CF_CHECK(hipGetLastError());
+}
+static void fill_u64(void* p, int64_t n)
+{
+    Engine& E = eng();
+    if (n <= 0) return;
+    k_fill_u64<<<(unsigned)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, E.stream>>>(
+        n, (unsigned long long*)p, 0ull);
+    CF_CHECK(hipGetLastError());
 }
 
 // -------------------------------------------------- bins (utils.c:62-106)
@@ -243,7 +313,7 @@ static void device_scan(const int* in, int* out, int64_t n, const std::string& t
 {
     Engine& E = eng();
     if (n <= 0) {
-        CF_CHECK(hipMemsetAsync(out, 0, sizeof(int), E.stream));
+        fill_i32(out, 1, 0);
         return;
     }
     const int64_t per = (int64_t)kScanBlock * kScanItems;
@@ -335,8 +405,8 @@ static int gridlink_launch(const std::string& tag, const GridSpec& g, int64_t n,
     L.qinv = (double)kQMax / (fmax(g.xbs, fmax(g.ybs, g.zbs)) + g.qreach);
     L.q = (uint2*)E.arena.get(tag + "_q", sizeof(uint2) * (n + kQPad));
     L.bounds = (double*)E.arena.get(tag + "_bounds", sizeof(double) * 6 * L.ncells);
-    CF_CHECK(hipMemsetAsync(counts, 0, sizeof(int) * (nb + 1), E.stream));
-    CF_CHECK(hipMemsetAsync(fill, 0, sizeof(int) * (nb + 1), E.stream));
+    fill_i32(counts, nb + 1, 0);
+    fill_i32(fill, nb + 1, 0);
     GridArgs ga{g.xmin, g.ymin, g.zmin, g.xinv, g.yinv, g.zinv, g.xbs, g.ybs, g.zbs, g.nx, g.ny, g.nz, L.qinv};
     const unsigned blocks = (unsigned)((n + 255) / 256);
     if (n > 0) {
@@ -380,7 +450,7 @@ static int gridlink_device(const std::string& tag, const GridSpec& g, int64_t n,
 {
     Engine& E = eng();
     unsigned long long* oob = (unsigned long long*)E.arena.get(tag + "_oob", 8);
-    CF_CHECK(hipMemsetAsync(oob, 0, 8, E.stream));
+    fill_u64(oob, 1);
     if (gridlink_launch(tag, g, n, X, Y, Z, W, esize, L, oob) != EXIT_SUCCESS) return EXIT_FAILURE;
     unsigned long long h_oob = 0;
     CF_CHECK(hipMemcpyAsync(&h_oob, oob, 8, hipMemcpyDeviceToHost, E.stream));
@@ -457,7 +527,7 @@ static int cellpairs_device(const DevLattice& L1, const DevLattice& L2, const in
     CF_CHECK(hipStreamSynchronize(E.stream));
     CellPairRec* cps = (CellPairRec*)E.arena.get("cp_list", sizeof(CellPairRec) * (ncp + 1));
     double* dtested = (double*)E.arena.get("cp_tested", sizeof(double));
-    CF_CHECK(hipMemsetAsync(dtested, 0, sizeof(double), E.stream));
+    fill_u64(dtested, 1);
     cellpairs_emit(cp, cps, dtested, cat);
     CF_CHECK(hipMemcpyAsync(tested, dtested, sizeof(double), hipMemcpyDeviceToHost, E.stream));
     CF_CHECK(hipStreamSynchronize(E.stream));
@@ -1376,7 +1446,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     CF_CHECK(hipMemcpyAsync(R.wavg.data(), g_w, sizeof(double) * nslots, hipMemcpyDeviceToHost, E.stream));
     if (need_w) {
         double* ws = (double*)E.arena.get("wsums", sizeof(double) * 2);
-        CF_CHECK(hipMemsetAsync(ws, 0, sizeof(double) * 2, E.stream));
+        fill_u64(ws, 2);
         if (c.N1 > 0) k_weight_sums<<<256, 256, 0, E.stream>>>(c.N1, L1.w, ws);
         CF_CHECK(hipGetLastError());
         double h[2];
@@ -1491,7 +1561,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     double wrap0[3] = {0, 0, 0}, coord_scale = 0, cs_max = 0, tested_all = 0;
     int rmaxr = 1;
     unsigned long long* d_oob = (unsigned long long*)E.arena.get("b_oob", sizeof(unsigned long long));
-    CF_CHECK(hipMemsetAsync(d_oob, 0, sizeof(unsigned long long), E.stream));
+    fill_u64(d_oob, 1);
     for (int k = 0; k < ncat; k++) {
         struct config_options oc = *options;
         const double* ext = &exts[6 * (size_t)k];
@@ -1571,7 +1641,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     // concatenated work list, emitted in place
     CellPairRec* cps = (CellPairRec*)E.arena.get("bcp_all", sizeof(CellPairRec) * ((size_t)ncp_all + 1));
     double* d_tested = (double*)E.arena.get("b_tested", sizeof(double) * ncat);
-    CF_CHECK(hipMemsetAsync(d_tested, 0, sizeof(double) * ncat, E.stream));
+    fill_u64(d_tested, ncat);
     for (int k = 0; k < ncat; k++) {
         if (ncps[k] == 0) continue;
         CpPlan cp;
@@ -1636,7 +1706,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     std::vector<double> wsums(2 * (size_t)ncat, 0.0);
     if (need_w) {
         double* ws = (double*)E.arena.get("bwsums", sizeof(double) * 2 * ncat);
-        CF_CHECK(hipMemsetAsync(ws, 0, sizeof(double) * 2 * ncat, E.stream));
+        fill_u64(ws, 2 * (int64_t)ncat);
         for (int k = 0; k < ncat; k++)
             if (ND[k] > 0) k_weight_sums<<<64, 256, 0, E.stream>>>(ND[k], hl[k].w, ws + 2 * k);
         CF_CHECK(hipGetLastError());
@@ -2205,6 +2275,93 @@ int countpairs(const int64_t ND1, void* X1, void* Y1, void* Z1, const int64_t ND
             free_results(results);
             return EXIT_FAILURE;
         }
+        for (int i = 0; i < nbin; i++) {
+            results->npairs[i] = R.np[i];
+            results->rupp[i] = R.rupp[i];
+            results->rpavg[i] = options->need_avg_sep ? R.avg[i] : 0.0;
+            results->weightavg[i] = R.need_w ? R.wavg[i] : 0.0;
+        }
+        reset_bin_refine_factors(options);
+        if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+// countpairs() with caller-owned results and no device allocation: every
+// buffer the call's kernels use lives in the engine's fixed scratch array
+// (kScratchBytes per GPU). Calls that overlap on one GPU race on it; the
+// engine's lock serialises calls from threads of one process.
+int cf_amd_countpairs_noalloc(const int64_t ND1, void* X1, void* Y1, void* Z1, const int64_t ND2, void* X2, void* Y2,
+                              void* Z2, const int numthreads, const int autocorr, const char* binfile,
+                              results_countpairs* results, struct config_options* options,
+                              struct extra_options* extra)
+{
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        int dev = -1;
+        CF_CHECK(hipGetDevice(&dev));
+        if (dev != E.device || E.scratch_ptr == nullptr) {
+            fprintf(stderr, "Error: cf_amd_countpairs_noalloc runs on the engine's device %d (current %d) with its "
+                            "scratch array\n", E.device, dev);
+            return EXIT_FAILURE;
+        }
+        if (results == NULL || results->npairs == NULL || results->rupp == NULL || results->rpavg == NULL ||
+            results->weightavg == NULL) {
+            fprintf(stderr, "Error: cf_amd_countpairs_noalloc writes into the caller's results arrays "
+                            "(npairs, rupp, rpavg, weightavg of results->nbin entries)\n");
+            return EXIT_FAILURE;
+        }
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        const int need_w = extra->weight_method != NONE;
+        const void* ins[8] = {X1, Y1, Z1, need_w ? extra->weights0.weights[0] : nullptr, nullptr, nullptr, nullptr,
+                              nullptr};
+        if (!autocorr) {
+            ins[4] = X2; ins[5] = Y2; ins[6] = Z2;
+            ins[7] = need_w ? extra->weights1.weights[0] : nullptr;
+        }
+        for (int k = 0; k < 8; k++) {
+            const int64_t n = k < 4 ? ND1 : ND2;
+            if (ins[k] && n > 0 && !is_device_ptr(ins[k])) {
+                fprintf(stderr, "Error: cf_amd_countpairs_noalloc takes positions and weights in device memory\n");
+                return EXIT_FAILURE;
+            }
+        }
+        if (E.model.recording) {
+            fprintf(stderr, "Error: cost recording allocates its sample store; turn it off for "
+                            "cf_amd_countpairs_noalloc\n");
+            return EXIT_FAILURE;
+        }
+        const int cap = results->nbin;
+        const double t0 = now_s();
+        fix_maxcells(options);
+        fix_refine(options);
+        options->sort_on_z = 1;
+        CallSpec c{MODE_DD, ND1, ND2, X1, Y1, Z1, X2, Y2, Z2, autocorr, binfile, 0, 0, 0, 0, numthreads};
+        CallOut R;
+        // the autotuner's candidate runs are left out: they would grow the scratch use
+        const bool autotune = E.autotune;
+        E.autotune = false;
+        E.arena.begin_fixed(E.scratch_ptr, kScratchBytes);
+        int rc = EXIT_FAILURE;
+        try {
+            rc = run_core(c, options, extra, R);
+        } catch (...) {
+            E.arena.end_fixed();
+            E.autotune = autotune;
+            throw;
+        }
+        E.arena.end_fixed();
+        E.autotune = autotune;
+        if (rc != EXIT_SUCCESS) return EXIT_FAILURE;
+        finalize(R, autocorr, ND1, options->need_avg_sep, 1);
+        const int nbin = R.nbin;
+        if (nbin > cap) {
+            fprintf(stderr, "Error: %d bin edges do not fit the caller's results arrays of %d entries\n", nbin, cap);
+            return EXIT_FAILURE;
+        }
+        results->nbin = nbin;
         for (int i = 0; i < nbin; i++) {
             results->npairs[i] = R.np[i];
             results->rupp[i] = R.rupp[i];

@@ -131,6 +131,30 @@ int cf_amd_countpairs_batch(int ncat, const int64_t *ND, void **X, void **Y, voi
                             const int numthreads, const char *binfile, results_countpairs *results,
                             struct config_options *options, struct extra_options *extra);
 
+/* countpairs() for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). It has
+ * no counterpart in the reference. Same arguments and results as countpairs()
+ * (DD(r), theory/DD/countpairs.h:28-35) with three differences:
+ *  - positions and weights must be in GPU memory;
+ *  - results is the caller's: results->npairs, rupp, rpavg and weightavg point
+ *    to arrays of results->nbin entries on entry (at least the number of bin
+ *    edges); on return results->nbin holds the number of edges. Nothing is
+ *    malloc'ed: do not call free_results on it;
+ *  - every buffer the kernels use lives in one fixed device array of
+ *    CF_AMD_NOALLOC_SCRATCH_BYTES per GPU, held for the life of the process:
+ *    about 70 B per particle of each set and 64 B per neighbour-cell pair (about
+ *    38 per cell in an autocorrelation, 75 in a cross-correlation, at the
+ *    default refine factors); a call that needs more fails with a message.
+ *    The call allocates no device or pinned memory.
+ * The scratch array is shared by every call on a GPU: calls that overlap on one
+ * GPU race (the library serialises calls from threads of one process).
+ * The autotuner and cost recording are not used by this entry point. */
+#define CF_AMD_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cf_amd_countpairs_noalloc(const int64_t ND1, void *X1, void *Y1, void *Z1, const int64_t ND2, void *X2,
+                              void *Y2, void *Z2, const int numthreads, const int autocorr, const char *binfile,
+                              results_countpairs *results, struct config_options *options,
+                              struct extra_options *extra);
+
 /* Device memory on the engine's GPU, from the engine's own HIP runtime, for
  * callers that keep catalogs resident in HBM across calls. */
 void *cf_amd_device_alloc(size_t bytes);            /* NULL on failure */
