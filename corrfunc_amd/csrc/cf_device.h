@@ -93,6 +93,7 @@ struct PairParams {
     int fetch;               // work items per queue atomic outside the single-fetch ranges
     int prim_cache;          // 1: the sparse path keeps the wave's primaries in LDS for its drains
     int max_window;          // longest secondary window scanned with one queue base (2^25, a multiple of 4)
+    int qblock_log2;         // work items go to sub-queues in blocks of 2^qblock_log2 consecutive items
 };
 
 // Work item of the tuned kernel: (cell pair, <= 64 primary particles, one part

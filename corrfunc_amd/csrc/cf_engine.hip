@@ -772,6 +772,11 @@ static void build_params(int mode, int isa_id, int nbin, int n2, int nslots, con
         // window changes; a multiple of 4 below 2^26)
         const char* mw = getenv("CORRFUNC_AMD_MAX_WINDOW");
         P.max_window = mw ? std::max(4, std::min(1 << 25, atoi(mw))) & ~3 : 1 << 25;
+        const char* qb = getenv("CORRFUNC_AMD_QBLOCK_LOG2");
+        // blocks of 32 items per sub-queue (measured, bench.py --sweep
+        // CORRFUNC_AMD_QBLOCK_LOG2=0..8): DDsmu cross -1.8 %, DDrppi -1.5 %, DD and xi
+        // within 1 % (the kernels are bound by instruction issue, not L2 misses)
+        P.qblock_log2 = qb ? std::max(0, std::min(12, atoi(qb))) : 5;
         const char* fe = getenv("CORRFUNC_AMD_FETCH");
         P.fetch = fe ? std::max(1, atoi(fe)) : 0;  // 0: chosen at launch from the work list
         if (eng().force_fetch > 0) P.fetch = eng().force_fetch;

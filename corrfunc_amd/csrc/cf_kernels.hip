@@ -1288,14 +1288,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const DevLattice* l1p = lats;
     const DevLattice* l2p = lats + (BATCH ? 0 : 1);
     int cur = -1;  // catalog whose histogram the wave's LDS region holds (BATCH)
-    // Work queue: item g belongs to sub-queue g % kQueues (the cost order is kept
-    // inside every sub-queue); sub-queue q has its own counter at
-    // counter[q * kCounterStride], so the device-wide atomics spread over
-    // kQueues addresses. A wave starts at sub-queue blockIdx.x % kQueues and moves
-    // on to the next when that one is exhausted, until all are. Items below
-    // *single_p (the heavy cell pairs the scheduler puts first) are taken one at
-    // a time; the others P.fetch per atomic while more than 2 P.fetch items per
-    // wave of the grid remain (as last seen), then one at a time.
+    // Work queue: the list is cut into blocks of B = 2^qblock_log2 consecutive
+    // items, and block b belongs to sub-queue b % kQueues (the cost order is kept
+    // inside every sub-queue): the k-th item of sub-queue q is
+    // g = ((k / B) * kQueues + q) * B + k % B. Sub-queue q has its own counter at
+    // counter[q * kCounterStride], so the device-wide atomics spread over kQueues
+    // addresses. A wave starts at sub-queue blockIdx.x % kQueues -- workgroups are
+    // dispatched to the XCDs round-robin, so that is mostly the wave's own XCD --
+    // and moves on to the next when that one is exhausted, until all are. Blocks
+    // keep the items of a cell pair, which scan the same secondary window, on one
+    // XCD and its L2. Items below *single_p (the heavy cell pairs the scheduler
+    // puts first) are taken one at a time; the others P.fetch per atomic while
+    // more than 2 P.fetch items per wave of the grid remain (as last seen), then
+    // one at a time.
     // The loop has no lane-divergent branch around its wave-wide steps: every
     // lane executes the queue atomic (lane 0 adds the batch, the others 0) and
     // the broadcast reads lane 0's result. A divergent `if (lane == 0)` here lets
@@ -1304,19 +1309,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int tail_zone = (int)min((long)nitems, (long)gridDim.x * nwaves * P.fetch * 2);
     const int n_single = *single_p;
     const int home = blockIdx.x & (kQueues - 1);
-    int qi = 0, next = 0, end = 0, last = 0;
+    const int lb = P.qblock_log2, bmask = (1 << lb) - 1;
+    const long round = (long)kQueues << lb;  // items of one block per sub-queue
+    int qi = 0, q = 0, next = 0, end = 0, last = 0;
     for (;;) {
         if (next >= end) {
             bool got = false;
             while (qi < kQueues) {
-                const int q = (home + qi) & (kQueues - 1);
-                const int cnt = nitems > q ? (nitems - q + kQueues - 1) / kQueues : 0;
+                q = (home + qi) & (kQueues - 1);
+                const long full = (long)nitems / round;
+                const long rem = (long)nitems - full * round - ((long)q << lb);
+                const int cnt = (int)((full << lb) + (rem < 0 ? 0 : (rem > bmask ? bmask + 1 : rem)));
                 const int want = (last >= n_single && last < nitems - tail_zone) ? P.fetch : 1;
                 const int fetched = atomicAdd(&counter[q * kCounterStride], lane == 0 ? want : 0);
                 const int k = __builtin_amdgcn_readlane(fetched, 0);
                 if (k < cnt) {
-                    next = q + kQueues * k;
-                    end = q + kQueues * min(k + want, cnt);
+                    next = k;
+                    end = min(k + want, cnt);
                     got = true;
                     break;
                 }
@@ -1324,8 +1333,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
             }
             if (!got) break;
         }
-        const int item = next;
-        next += kQueues;
+        const int item = (((next >> lb) * kQueues + q) << lb) + (next & bmask);
+        next++;
         last = item;
         const WorkItem it = items[item];
         const CellPairRec* cpp = cps + __builtin_amdgcn_readfirstlane(it.cp);

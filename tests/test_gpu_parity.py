@@ -147,6 +147,9 @@ KNOBS = [
     {"CORRFUNC_AMD_PRIM_CACHE": "1", "CORRFUNC_AMD_HIST_COPIES": "1"},
     {"CORRFUNC_AMD_SPLIT": "64", "CORRFUNC_AMD_HEAVY": "1"},
     {"CORRFUNC_AMD_SPLIT": "0", "CORRFUNC_AMD_DENSE": "2"},
+    {"CORRFUNC_AMD_QBLOCK_LOG2": "5"},
+    {"CORRFUNC_AMD_QBLOCK_LOG2": "3", "CORRFUNC_AMD_FETCH": "4", "CORRFUNC_AMD_HEAVY": "1e9"},
+    {"CORRFUNC_AMD_QBLOCK_LOG2": "12", "CORRFUNC_AMD_SPLIT": "64", "CORRFUNC_AMD_HEAVY": "1"},
 ]
 
 
