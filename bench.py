@@ -265,11 +265,20 @@ class W3(Workload):
         from corrfunc_amd import _abi
         self.binpath, _ = _abi.write_bin_file(self.bins)
         n = sum(len(c[0]) for c in self.cats)
+        self.graph = None
+        if args.graph:
+            # captured once (untimed), replayed each step on the same device arrays
+            self.graph = _lib.BatchGraph([tuple(d) for d in self.dev], self.bins,
+                                         options=_opts(periodic=True, boxsize=500.0, need_avg_sep=True,
+                                                       isa="fastest"))
         self.description = ("batched DD autocorrelation, 256 uniform float32 catalogs (%d points, sizes "
-                            "log-uniform 50k-2M), periodic L=500, 18 log r bins 0.5-50, one batched call" % n)
+                            "log-uniform 50k-2M), periodic L=500, 18 log r bins 0.5-50, one batched call%s"
+                            % (n, ", replayed from a captured HIP graph" if args.graph else ""))
 
     def step(self):
         from corrfunc_amd import _lib
+        if self.graph is not None:
+            return self.graph.launch(), [_lib.last_stats()]
         outs = _lib.countpairs_batch(self.dev, self.bins, options=_opts(periodic=True, boxsize=500.0,
                                                                         need_avg_sep=True, isa="fastest"))
         return outs, [_lib.last_stats()]
@@ -500,6 +509,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the workload's point count (smaller test runs)")
     ap.add_argument("--sched", default="learned", choices=["none", "heuristic", "learned"])
     ap.add_argument("--sched-ab", action="store_true", help="also time the pair kernel under every scheduler")
+    ap.add_argument("--graph", action="store_true",
+                    help="workload 3: capture the batch once into a HIP graph and time its replays")
     ap.add_argument("--autotune", action="store_true",
                     help="tuned form with the engine's autotuner (refine factors, LDS replication); the pairs "
                          "tested stay those of the default options, counted by an untimed default step")

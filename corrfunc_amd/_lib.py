@@ -140,6 +140,15 @@ class DeviceArray:
             raise RuntimeError("copy from device failed")
         return out
 
+    def upload(self, a):
+        """Overwrites the device copy with host array `a` (same size and dtype)."""
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch: %d vs %d bytes" % (a.nbytes, self.nbytes))
+        if lib().cf_amd_memcpy(self.ptr, a.ctypes.data, self.nbytes, 0) != 0:
+            raise RuntimeError("copy to device failed")
+
     def copy_from(self, other):
         """Device-to-device copy of another DeviceArray of the same size (the
         engine's own copy kernel)."""
@@ -205,3 +214,78 @@ def countpairs_batch(catalogs, binfile, options=None, weights=None, nthreads=1):
         lib().free_results(C.byref(res[k]))
         out.append(d)
     return out
+
+
+class BatchGraph:
+    """The batched DD(r) captured once into a HIP graph (cf_amd_batch_graph_*)
+    and replayed on whatever the catalogs' device arrays hold at each launch:
+    write resampled positions of the same sizes into the same DeviceArrays and
+    call launch() again.
+
+    catalogs: sequence of (X, Y, Z) DeviceArray triples; weights: optional
+    sequence of DeviceArray weights (pair_product). The options must give a
+    periodic box size."""
+
+    def __init__(self, catalogs, binfile, options=None, weights=None, nthreads=1):
+        import os
+        from . import _abi
+        L = lib()
+        if not hasattr(L, "_graph_bound"):
+            L.cf_amd_batch_graph_create.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_void_p),
+                                                    C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                                    C.POINTER(C.c_void_p), C.c_int, C.c_char_p,
+                                                    C.POINTER(_abi.config_options), C.POINTER(_abi.extra_options),
+                                                    C.POINTER(C.c_void_p)]
+            L.cf_amd_batch_graph_create.restype = C.c_int
+            L.cf_amd_batch_graph_launch.argtypes = [C.c_void_p, C.POINTER(_abi.results_countpairs)]
+            L.cf_amd_batch_graph_launch.restype = C.c_int
+            L.cf_amd_batch_graph_destroy.argtypes = [C.c_void_p]
+            L.cf_amd_batch_graph_destroy.restype = C.c_int
+            L._graph_bound = True
+        self.ncat = ncat = len(catalogs)
+        self.catalogs, self.weights = catalogs, weights  # keep the device arrays alive
+        ND = (C.c_int64 * ncat)()
+        Xs, Ys, Zs, Ws = [(C.c_void_p * ncat)() for _ in range(4)]
+        for k, (x, y, z) in enumerate(catalogs):
+            if not isinstance(x, DeviceArray):
+                raise TypeError("a graph batch reads its catalogs from DeviceArrays")
+            ND[k] = x.shape[0]
+            Xs[k], Ys[k], Zs[k] = x.ptr, y.ptr, z.ptr
+            if weights is not None:
+                Ws[k] = weights[k].ptr
+        o = options if options is not None else _abi.make_options()
+        o.float_type = catalogs[0][0].dtype.itemsize
+        extra = _abi.get_extra_options(_abi.WEIGHT_PAIR_PRODUCT if weights is not None else _abi.WEIGHT_NONE)
+        path, tmp = _abi.write_bin_file(binfile)
+        self.handle = C.c_void_p()
+        try:
+            st = L.cf_amd_batch_graph_create(ncat, ND, Xs, Ys, Zs, Ws, int(nthreads), path.encode(), C.byref(o),
+                                             C.byref(extra), C.byref(self.handle))
+        finally:
+            if tmp:
+                os.remove(path)
+        if st != 0:
+            raise RuntimeError("cf_amd_batch_graph_create failed")
+
+    def launch(self):
+        """Replays the graph; returns a list of dicts with the countpairs outputs of each catalog."""
+        from . import _abi
+        res = (_abi.results_countpairs * self.ncat)()
+        if lib().cf_amd_batch_graph_launch(self.handle, res) != 0:
+            raise RuntimeError("cf_amd_batch_graph_launch failed")
+        out = []
+        for k in range(self.ncat):
+            out.append(_abi._collect("countpairs", res[k]))
+            lib().free_results(C.byref(res[k]))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().cf_amd_batch_graph_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -29,6 +30,7 @@
 #include "cf_kernels.hip"
 #include "cf_sched.hip"
 #include "cf_model.hip"
+#include "cf_bgraph.hip"
 #include "../../include/corrfunc_theory.h"
 #include "../../include/corrfunc_amd.h"
 #include "../../include/corrfunc_mocks.h"
@@ -844,15 +846,18 @@ static void ensure_sample_store()
     CF_CHECK(hipGetLastError());
 }
 
-// Builds the work list of `ncp` cell pairs (ordered and split by the selected
-// scheduler) and launches the tuned pair kernel on it. `batch`: cell pair k
-// belongs to catalog cps[k].pad, whose lattice is lats[pad] and whose histogram
-// is g_*[pad * nslots]. Records E.ev_list before the work list, E.ev0 / E.ev1
-// around the pair kernel. With `timed`, per-cell-pair ticks are recorded (and
-// appended to the cost model's samples when recording is on).
-static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats, int batch, PairParams& P, int nslots,
-                        int nbin, const double* d_edges, const unsigned short* d_lut, unsigned long long* g_np,
-                        double* g_avg, double* g_w, bool timed, TunedRun& T)
+// The tuned pair kernel for P's mode, rounding order and `batch`, and its
+// launch shape (see the comment in the body); sets P.hist_copies and
+// P.prim_cache. Returns kNoTunedFit when even a one-copy layout does not fit a
+// workgroup's LDS.
+struct TunedShape {
+    const void* kfn = nullptr;
+    bool gh = false;
+    int threads = 0, nwaves = 0, per_cu = 0;
+    size_t lds = 0;
+};
+
+static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShape& out)
 {
     Engine& E = eng();
     const size_t per_slot = 8 + (P.need_avg ? 8 : 0) + (P.need_w ? 8 : 0);
@@ -945,18 +950,44 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
         if (const char* wg = getenv("CORRFUNC_AMD_THREADS"))
             sh = atoi(wg) == 512 && s8.lds <= E.lds_max ? s8 : shape_for(256);
     }
-    const int threads = sh.threads, nwaves = sh.nwaves;
     P.hist_copies = sh.R;
     P.prim_cache = sh.prim;
-    const size_t lds = sh.lds;
-    if (lds > E.lds_max) {
+    if (sh.lds > E.lds_max) {
         fprintf(stderr, "Error: LDS requirement %zu B of the pair kernel exceeds the %zu B a workgroup can have\n",
-                lds, E.lds_max);
+                sh.lds, E.lds_max);
         return EXIT_FAILURE;
     }
-    if (lds > 64 * 1024)
-        CF_CHECK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int per_cu = sh.per_cu;
+    if (sh.lds > 64 * 1024)
+        CF_CHECK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
+    out.kfn = kfn;
+    out.gh = gh;
+    out.threads = sh.threads;
+    out.nwaves = sh.nwaves;
+    out.per_cu = sh.per_cu;
+    out.lds = sh.lds;
+    return EXIT_SUCCESS;
+}
+
+
+// Builds the work list of `ncp` cell pairs (ordered and split by the selected
+// scheduler) and launches the tuned pair kernel on it. `batch`: cell pair k
+// belongs to catalog cps[k].pad, whose lattice is lats[pad] and whose histogram
+// is g_*[pad * nslots]. Records E.ev_list before the work list, E.ev0 / E.ev1
+// around the pair kernel. With `timed`, per-cell-pair ticks are recorded (and
+// appended to the cost model's samples when recording is on).
+static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats, int batch, PairParams& P, int nslots,
+                        int nbin, const double* d_edges, const unsigned short* d_lut, unsigned long long* g_np,
+                        double* g_avg, double* g_w, bool timed, TunedRun& T)
+{
+    Engine& E = eng();
+    TunedShape ts;
+    const int src = tuned_shape(batch, P, nslots, nbin, ts);
+    if (src != EXIT_SUCCESS) return src;
+    const void* kfn = ts.kfn;
+    const bool gh = ts.gh;
+    const int threads = ts.threads, nwaves = ts.nwaves;
+    const size_t lds = ts.lds;
+    const int per_cu = ts.per_cu;
     const long max_blocks = (long)E.ncu * per_cu;
 
     // ---- cost prediction and order
@@ -1747,6 +1778,469 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     st.queue_fetch = T.fetch;
     st.autotuned = E.last_autotuned;
     st.sched = E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC : E.sched_kind;
+    E.last = st;
+    return EXIT_SUCCESS;
+}
+
+// ---------------------------------------------------------- graph batches
+// A batch of DD(r) autocorrelations captured once into a hipGraph and replayed
+// on the current contents of the catalogs' device arrays (cf_bgraph.hip for
+// the device side). The capture runs the engine's regular batch once on the
+// capture-time positions to size the cell-pair list and the work list (with a
+// margin); every other buffer is sized from the box.
+struct BatchGraph {
+    int ncat = 0;
+    std::vector<int64_t> nd;
+    size_t esize = 8;
+    int need_w = 0, need_avg = 0;
+    int nbin = 0;
+    std::vector<double> rupp;
+    std::vector<void*> owned;            // device buffers of the graph
+    std::vector<int64_t> ncells_cap;
+    int pairs_cap = 0, items_cap = 0;
+    unsigned long long* g_np = nullptr;
+    double *g_avg = nullptr, *g_w = nullptr, *ws = nullptr, *tested = nullptr;
+    int *fail = nullptr, *cp_off = nullptr, *nitems_eff = nullptr;
+    BgCat* cats = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int hist_copies = 0, queue_fetch = 0, sched = CF_AMD_SCHED_NONE;
+    void* alloc(size_t bytes)
+    {
+        void* p = nullptr;
+        CF_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+        owned.push_back(p);
+        return p;
+    }
+    ~BatchGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : owned) (void)hipFree(p);
+    }
+};
+
+// Exclusive scan with block-sum buffers allocated at capture (no arena:
+// buffers a captured graph reads must outlive any later arena regrowth).
+struct GraphScan {
+    std::vector<std::pair<int*, int*>> lv;  // per level: block sums, block offsets
+    void init(int64_t nmax, BatchGraph& G)
+    {
+        const int64_t per = (int64_t)kScanBlock * kScanItems;
+        int64_t n = nmax > 0 ? nmax : 1;
+        for (;;) {
+            const int64_t nb = (n + per - 1) / per;
+            lv.push_back({(int*)G.alloc(sizeof(int) * (nb + 1)), (int*)G.alloc(sizeof(int) * (nb + 1))});
+            if (nb <= 1) break;
+            n = nb;
+        }
+    }
+    void run(const int* in, int* out, int64_t n, hipStream_t st, size_t level = 0)
+    {
+        if (n <= 0) {
+            k_fill_i32<<<1, 64, 0, st>>>(1, out, 0);
+            return;
+        }
+        const int64_t per = (int64_t)kScanBlock * kScanItems;
+        const int64_t nblocks = (n + per - 1) / per;
+        k_scan_blocks<<<(unsigned)nblocks, kScanBlock, 0, st>>>(n, in, out, lv[level].first);
+        if (nblocks > 1) {
+            run(lv[level].first, lv[level].second, nblocks, st, level + 1);
+            k_scan_add<<<(unsigned)nblocks, kScanBlock, 0, st>>>(n, out, lv[level].second);
+            k_copy_int<<<1, 64, 0, st>>>(lv[level].second + nblocks, out + n);
+        } else {
+            k_copy_int<<<1, 64, 0, st>>>(lv[level].first, out + n);
+        }
+    }
+};
+
+static void fix_refine(struct config_options* o);
+static void fix_maxcells(struct config_options* o);
+
+static int bg_create(int ncat, const int64_t* ND, void** X, void** Y, void** Z, void** W, const char* binfile,
+                     struct config_options* options, struct extra_options* extra, BatchGraph& G)
+{
+    Engine& E = eng();
+    const size_t esize = options->float_type;
+    const int need_w = extra->weight_method != NONE;
+    if (need_w && extra->weight_method != PAIR_PRODUCT) {
+        fprintf(stderr, "Error: unknown weight method %d\n", (int)extra->weight_method);
+        return EXIT_FAILURE;
+    }
+    // the grid capacities come from the box: a periodic box with its size given
+    const double by = options->boxsize_y == BOXSIZE_NOTGIVEN ? options->boxsize : options->boxsize_y;
+    const double bz = options->boxsize_z == BOXSIZE_NOTGIVEN ? options->boxsize : options->boxsize_z;
+    const double wrap[3] = {options->boxsize_x, by, bz};
+    if (!options->periodic || !(wrap[0] > 0 && wrap[1] > 0 && wrap[2] > 0)) {
+        fprintf(stderr, "Error: a graph batch needs a periodic box with its size given on every axis\n");
+        return EXIT_FAILURE;
+    }
+    for (int k = 0; k < ncat; k++) {
+        const void* ins[4] = {X[k], Y[k], Z[k], need_w ? W[k] : nullptr};
+        for (int d = 0; d < 4; d++)
+            if (ND[k] > 0 && (d < 3 || need_w) && (ins[d] == nullptr || !is_device_ptr(ins[d]))) {
+                fprintf(stderr, "Error: a graph batch reads positions%s from device memory (catalog %d)\n",
+                        need_w ? " and weights" : "", k);
+                return EXIT_FAILURE;
+            }
+    }
+    // the regular batch on the capture-time positions sizes the two lists
+    {
+        struct config_options oc = *options;
+        std::vector<CallOut> outs;
+        const int sched = E.sched_kind;
+        if (run_batch(ncat, ND, X, Y, Z, W, binfile, &oc, extra, outs) != EXIT_SUCCESS) return EXIT_FAILURE;
+        (void)sched;
+    }
+    const double ncp0 = (double)E.last.num_cell_pairs, items0 = (double)E.last.num_work_items;
+    const int fetch0 = E.last.queue_fetch;
+    std::vector<double> rupp;
+    double rmin = 0, rmax = 0;
+    int nbin = 0;
+    if (setup_bins(binfile, &rmin, &rmax, &nbin, rupp) != EXIT_SUCCESS) return EXIT_FAILURE;
+    std::vector<double> rupp_sqr(nbin);
+    for (int i = 0; i < nbin; i++) rupp_sqr[i] = rupp[i] * rupp[i];
+    int sorted = 1;
+    for (int i = 1; i + 1 < nbin; i++)
+        if (!(rupp_sqr[i] >= rupp_sqr[i - 1])) sorted = 0;
+    G.ncat = ncat;
+    G.nd.assign(ND, ND + ncat);
+    G.esize = esize;
+    G.need_w = need_w;
+    G.need_avg = options->need_avg_sep;
+    G.nbin = nbin;
+    G.rupp = rupp;
+    const int nslots = nbin;
+
+    BgOpts bo;
+    memset(&bo, 0, sizeof bo);
+    bo.rmax = rmax;
+    bo.sqr_rmax = rupp_sqr[nbin - 1];
+    bo.sqr_rmin = rupp_sqr[0];
+    int refine_cap[3];
+    {
+        struct config_options oc = *options;
+        fix_maxcells(&oc);
+        fix_refine(&oc);
+        bo.custom = get_bin_refine_scheme(&oc) != BINNING_DFL;
+        bo.max_cells = oc.max_cells_per_dim;
+        for (int d = 0; d < 3; d++) {
+            bo.wrap[d] = wrap[d];
+            bo.per[d] = 1;
+            bo.refine[d] = oc.bin_refine_factors[d];
+            int r = bo.refine[d];
+            if (!bo.custom && rmax < 0.05 * wrap[d]) r = 1;
+            if (!bo.custom && d < 2) r += 1;  // the boost
+            refine_cap[d] = std::max(r, bo.refine[d] + (d < 2 && !bo.custom ? 1 : 0));
+        }
+    }
+    bo.min_sep_opt = options->enable_min_sep_opt;
+    // cells: nmesh = (int)(refine * extent / rmax) with the extent within the
+    // box, plus one for positions on its faces
+    int64_t ncap = 1;
+    for (int d = 0; d < 3; d++) {
+        int64_t m = (int64_t)(refine_cap[d] * wrap[d] / rmax) + 1;
+        m = std::max<int64_t>(2, std::min<int64_t>(m, bo.max_cells));
+        ncap *= m;
+    }
+    const int64_t nngb_cap = (int64_t)(2 * refine_cap[0] + 1) * (2 * refine_cap[1] + 1) * (2 * refine_cap[2] + 1);
+    const int64_t cand_cap = ncap * nngb_cap;
+    if (ncap * kSlabs >= (int64_t)INT32_MAX || cand_cap >= (int64_t)INT32_MAX) {
+        fprintf(stderr, "Error: the graph batch's grid capacity exceeds the engine's 32-bit indexing\n");
+        return EXIT_FAILURE;
+    }
+    bo.ncells_cap = ncap;
+    G.pairs_cap = (int)std::min<double>((double)INT32_MAX - 1, ncp0 * 1.25 + 4096);
+    G.items_cap = (int)std::min<double>((double)INT32_MAX - 1, items0 * 1.25 + 4096);
+
+    // launch parameters, with the coordinate scale of a box whose positions lie
+    // within [0, wrap] (a wider edge window than the data's only sends more
+    // pairs through the pruning replay)
+    double coord_scale = 0, cs = 0;
+    for (int d = 0; d < 3; d++) {
+        coord_scale += 2 * wrap[d] + wrap[d];
+        cs = fmax(cs, wrap[d]);
+    }
+    coord_scale += rmax;
+    PairParams P;
+    std::vector<unsigned short> lut;
+    build_params(MODE_DD, resolve_isa(options->instruction_set), nbin, 0, nslots, rupp_sqr, sorted, rmax, rmax, 0,
+                 0, options->need_avg_sep, need_w, 1, wrap, coord_scale, cs, refine_cap[0], P, lut);
+    TunedShape ts;
+    const int src = tuned_shape(1, P, nslots, nbin, ts);
+    if (src == kNoTunedFit) {
+        fprintf(stderr, "Error: %d bins are more than the batched kernel's LDS holds\n", nbin - 1);
+        return EXIT_FAILURE;
+    }
+    if (src != EXIT_SUCCESS) return EXIT_FAILURE;
+    // the queue batch the regular batch chose for these catalogs (launch_tuned's rule)
+    if (P.fetch <= 0) P.fetch = fetch0 > 0 ? fetch0 : 1;
+    G.hist_copies = P.hist_copies;
+    G.queue_fetch = P.fetch;
+    const long blocks = (long)E.ncu * ts.per_cu;
+
+    // ---- buffers
+    const size_t tot = (size_t)ncat * nslots;
+    G.g_np = (unsigned long long*)G.alloc(sizeof(uint64_t) * tot);
+    G.g_avg = (double*)G.alloc(sizeof(double) * tot);
+    G.g_w = (double*)G.alloc(sizeof(double) * tot);
+    G.ws = (double*)G.alloc(sizeof(double) * 2 * (size_t)ncat);
+    G.tested = (double*)G.alloc(sizeof(double) * (size_t)ncat);
+    G.fail = (int*)G.alloc(sizeof(int));
+    G.cp_off = (int*)G.alloc(sizeof(int) * ((size_t)ncat + 1));
+    G.nitems_eff = (int*)G.alloc(sizeof(int));
+    G.cats = (BgCat*)G.alloc(sizeof(BgCat) * (size_t)ncat);
+    DevLattice* lats = (DevLattice*)G.alloc(sizeof(DevLattice) * (size_t)ncat);
+    double* ext = (double*)G.alloc(sizeof(double) * 6 * (size_t)ncat);
+    double* mm_part = (double*)G.alloc(sizeof(double) * 6 * 512);
+    unsigned long long* oob = (unsigned long long*)G.alloc(sizeof(unsigned long long));
+    int* flags = (int*)G.alloc(sizeof(int) * (size_t)(cand_cap + 1));
+    int* pos = (int*)G.alloc(sizeof(int) * (size_t)(cand_cap + 1));
+    CellPairRec* cps = (CellPairRec*)G.alloc(sizeof(CellPairRec) * ((size_t)G.pairs_cap + 1));
+    int* nch = (int*)G.alloc(sizeof(int) * ((size_t)G.pairs_cap + 1));
+    int* ioff = (int*)G.alloc(sizeof(int) * ((size_t)G.pairs_cap + 1));
+    WorkItem* items = (WorkItem*)G.alloc(sizeof(WorkItem) * ((size_t)G.items_cap + 1));
+    int* counter = (int*)G.alloc(sizeof(int) * kQueues * kCounterStride);
+    int* single = (int*)G.alloc(sizeof(int));
+    double* d_edges = (double*)G.alloc(sizeof(double) * nbin);
+    unsigned short* d_lut = (unsigned short*)G.alloc(sizeof(unsigned short) * lut.size());
+    CF_CHECK(hipMemcpy(d_edges, rupp_sqr.data(), sizeof(double) * nbin, hipMemcpyHostToDevice));
+    CF_CHECK(hipMemcpy(d_lut, lut.data(), sizeof(unsigned short) * lut.size(), hipMemcpyHostToDevice));
+    const int64_t nb_cap = ncap * kSlabs;
+    std::vector<DevLattice> tmpl(ncat);
+    std::vector<int*> bucket(ncat), counts(ncat), fillb(ncat);
+    for (int k = 0; k < ncat; k++) {
+        const int64_t n = ND[k];
+        DevLattice& L = tmpl[k];
+        memset(&L, 0, sizeof L);
+        L.x = (double*)G.alloc(sizeof(double) * (size_t)n);
+        L.y = (double*)G.alloc(sizeof(double) * (size_t)n);
+        L.z = (double*)G.alloc(sizeof(double) * (size_t)n);
+        L.w = need_w ? (double*)G.alloc(sizeof(double) * (size_t)n) : nullptr;
+        L.d4 = (double4*)G.alloc(sizeof(double4) * (size_t)(n + kQPad));
+        L.q = (uint2*)G.alloc(sizeof(uint2) * (size_t)(n + kQPad));
+        L.bstart = (int*)G.alloc(sizeof(int) * (size_t)(nb_cap + 1));
+        L.bounds = (double*)G.alloc(sizeof(double) * 6 * (size_t)ncap);
+        bucket[k] = (int*)G.alloc(sizeof(int) * (size_t)(n + 1));
+        counts[k] = (int*)G.alloc(sizeof(int) * (size_t)(nb_cap + 1));
+        fillb[k] = (int*)G.alloc(sizeof(int) * (size_t)(nb_cap + 1));
+    }
+    // scheduler buffers (as launch_tuned's)
+    const int sched = E.sched_kind;
+    const bool learned = sched == CF_AMD_SCHED_LEARNED && E.model.trained;
+    const int64_t nsb = ((int64_t)G.pairs_cap + kSortBlock - 1) / kSortBlock;
+    const int64_t nmat = (int64_t)kSortBuckets * nsb;
+    float* cost = (float*)G.alloc(sizeof(float) * (size_t)G.pairs_cap);
+    float* feats = learned ? (float*)G.alloc(sizeof(float) * kNumFeatures * (size_t)G.pairs_cap) : nullptr;
+    double* total = (double*)G.alloc(sizeof(double));
+    int* bcount = (int*)G.alloc(sizeof(int) * (size_t)(nmat + 1));
+    int* boff = (int*)G.alloc(sizeof(int) * (size_t)(nmat + 1));
+    int* order = (int*)G.alloc(sizeof(int) * (size_t)G.pairs_cap);
+    int* nparts = (int*)G.alloc(sizeof(int) * (size_t)G.pairs_cap);
+    const char* hv = getenv("CORRFUNC_AMD_HEAVY");
+    const float heavy = hv ? (float)atof(hv) : kHeavyFactor;
+    const char* sp = getenv("CORRFUNC_AMD_SPLIT");
+    const double split = sp ? atof(sp) : (double)kSplitPerWave;
+    const double cap_div = split > 0 ? (double)blocks * ts.nwaves * split : 1.0;
+    G.sched = learned ? CF_AMD_SCHED_LEARNED : (sched == CF_AMD_SCHED_NONE ? CF_AMD_SCHED_NONE
+                                                                           : CF_AMD_SCHED_HEURISTIC);
+    GraphScan scan;
+    scan.init(std::max<int64_t>(std::max<int64_t>(nb_cap, cand_cap), std::max<int64_t>(G.pairs_cap, nmat)), G);
+
+    // ---- the captured sequence (fill kernels only: no memset nodes)
+    hipStream_t st = E.stream;
+    auto fill_i = [&](int* p, int64_t n, int v) {
+        k_fill_i32<<<(unsigned)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, st>>>(n, p, v);
+    };
+    auto fill_z = [&](void* p, int64_t n) {
+        k_fill_u64<<<(unsigned)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, st>>>(
+            n, (unsigned long long*)p, 0ull);
+    };
+    CF_CHECK(hipStreamSynchronize(st));
+    CF_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    fill_i(G.fail, 1, 0);
+    fill_i(G.cp_off, ncat + 1, 0);
+    fill_z(G.tested, ncat);
+    fill_z(oob, 1);
+    fill_z(G.g_np, (int64_t)tot);
+    fill_z(G.g_avg, (int64_t)tot);
+    fill_z(G.g_w, (int64_t)tot);
+    fill_z(G.ws, 2 * (int64_t)ncat);
+    for (int k = 0; k < ncat; k++) {
+        // extent
+        if (esize == 4)
+            k_minmax<float><<<512, 256, 0, st>>>(ND[k], (const float*)X[k], (const float*)Y[k], (const float*)Z[k],
+                                                 mm_part);
+        else
+            k_minmax<double><<<512, 256, 0, st>>>(ND[k], (const double*)X[k], (const double*)Y[k],
+                                                  (const double*)Z[k], mm_part);
+        k_minmax_final<<<1, 64, 0, st>>>(512, mm_part, ext + 6 * k);
+        // grid
+        k_bg_grid<<<1, 64, 0, st>>>(k, bo, ext, ND[k], tmpl[k], G.cats, lats, G.fail);
+        // counting sort into the grid
+        const int64_t n = ND[k];
+        const BgCat* ck = G.cats + k;
+        fill_i(counts[k], nb_cap + 1, 0);
+        fill_i(fillb[k], nb_cap + 1, 0);
+        const unsigned nblk = (unsigned)((n + 255) / 256);
+        if (n > 0) {
+            if (esize == 4)
+                k_bg_bucket_count<float><<<nblk, 256, 0, st>>>(n, (const float*)X[k], (const float*)Y[k],
+                                                               (const float*)Z[k], ck, bucket[k], counts[k], oob);
+            else
+                k_bg_bucket_count<double><<<nblk, 256, 0, st>>>(n, (const double*)X[k], (const double*)Y[k],
+                                                                (const double*)Z[k], ck, bucket[k], counts[k], oob);
+        }
+        scan.run(counts[k], tmpl[k].bstart, nb_cap, st);
+        if (n > 0) {
+            if (esize == 4)
+                k_bg_scatter<float><<<nblk, 256, 0, st>>>(n, (const float*)X[k], (const float*)Y[k],
+                                                          (const float*)Z[k], (const float*)(need_w ? W[k] : nullptr),
+                                                          ck, bucket[k], tmpl[k].bstart, fillb[k], tmpl[k].x,
+                                                          tmpl[k].y, tmpl[k].z, tmpl[k].w, tmpl[k].d4, tmpl[k].q);
+            else
+                k_bg_scatter<double><<<nblk, 256, 0, st>>>(n, (const double*)X[k], (const double*)Y[k],
+                                                           (const double*)Z[k],
+                                                           (const double*)(need_w ? W[k] : nullptr), ck, bucket[k],
+                                                           tmpl[k].bstart, fillb[k], tmpl[k].x, tmpl[k].y,
+                                                           tmpl[k].z, tmpl[k].w, tmpl[k].d4, tmpl[k].q);
+        }
+        k_bg_cell_bounds<<<(unsigned)((ncap + 3) / 4), 256, 0, st>>>(ck, tmpl[k].bstart, tmpl[k].x, tmpl[k].y,
+                                                                      tmpl[k].z, tmpl[k].bounds);
+        // neighbour-cell pairs, appended to the list at cp_off[k]
+        k_bg_cellpair_flag<<<(unsigned)((cand_cap + 255) / 256), 256, 0, st>>>(ck, cand_cap, tmpl[k].bstart,
+                                                                                tmpl[k].bounds, flags);
+        scan.run(flags, pos, cand_cap, st);
+        k_bg_cp_offset<<<1, 64, 0, st>>>(k, pos, cand_cap, G.cp_off, G.pairs_cap, G.fail);
+        k_bg_cellpair_emit<<<(unsigned)std::min<int64_t>((cand_cap + 255) / 256, 1024), 256, 0, st>>>(
+            ck, cand_cap, tmpl[k].bstart, tmpl[k].bounds, pos, G.cp_off, k, G.pairs_cap, cps, G.tested);
+        if (need_w && n > 0) k_weight_sums<<<64, 256, 0, st>>>(n, tmpl[k].w, G.ws + 2 * k);
+    }
+    k_oob_flag<<<1, 64, 0, st>>>(oob, G.fail, kBgFailOob);
+    // work list as launch_tuned builds it (cost order, heavy cell pairs first
+    // and split), over the capacity with the device's cell-pair count
+    const int* ncp_dev = G.cp_off + ncat;
+    const unsigned pblk = (unsigned)(((int64_t)G.pairs_cap + 255) / 256);
+    int* nheavy = nullptr;
+    if (sched != CF_AMD_SCHED_NONE) {
+        if (learned) {
+            k_cp_features<<<pblk, 256, 0, st>>>(G.pairs_cap, cps, lats, 1, P, feats, ncp_dev);
+            k_mlp_predict<<<pblk, 256, 0, st>>>(G.pairs_cap, feats, E.model.d_norm, E.model.d_params, cost, ncp_dev);
+        } else {
+            k_cost_heuristic<<<pblk, 256, 0, st>>>(G.pairs_cap, cps, lats, 1, cost, ncp_dev);
+        }
+        fill_z(total, 1);
+        k_cost_total<<<std::min<unsigned>(pblk, 1024), 256, 0, st>>>(G.pairs_cap, cost, total, ncp_dev);
+        k_cost_hist<<<(unsigned)nsb, 256, 0, st>>>(G.pairs_cap, cost, total, heavy, (int)nsb, bcount, ncp_dev);
+        scan.run(bcount, boff, nmat, st);
+        k_cost_place<<<(unsigned)nsb, 256, 0, st>>>(G.pairs_cap, cost, total, heavy, (int)nsb, boff, order,
+                                                    ncp_dev);
+        nheavy = boff + (int64_t)(kSortBuckets - 1) * nsb;
+    }
+    k_count_chunks<<<pblk, 256, 0, st>>>(G.pairs_cap, sched != CF_AMD_SCHED_NONE ? order : nullptr, cps, lats, 1,
+                                         sched != CF_AMD_SCHED_NONE ? cost : nullptr, total, cap_div,
+                                         sched != CF_AMD_SCHED_NONE ? nparts : nullptr, nch, ncp_dev);
+    scan.run(nch, ioff, G.pairs_cap, st);
+    k_bg_items_clamp<<<1, 64, 0, st>>>(ioff + G.pairs_cap, G.items_cap, G.nitems_eff, G.fail);
+    k_emit_chunks<<<pblk, 256, 0, st>>>(G.pairs_cap, sched != CF_AMD_SCHED_NONE ? order : nullptr, cps, lats, 1, P,
+                                        ioff, sched != CF_AMD_SCHED_NONE ? nparts : nullptr, items, ncp_dev,
+                                        G.items_cap);
+    fill_i(counter, kQueues * kCounterStride, 0);
+    if (nheavy)
+        k_single_boundary<<<1, 64, 0, st>>>(nheavy, ioff, single);
+    else
+        fill_i(single, 1, INT32_MAX);
+    {
+        const int* d_nitems = G.nitems_eff;
+        unsigned long long* ticks = nullptr;
+        int* cu = nullptr;
+        const DevLattice* clats = lats;
+        const CellPairRec* ccps = cps;
+        const WorkItem* citems = items;
+        const double* cedges = d_edges;
+        const unsigned short* clut = d_lut;
+        const int* csingle = single;
+        void* kargs[] = {&P, &clats, &ccps, &citems, &d_nitems, &counter, &cedges, &clut, &G.g_np, &G.g_avg, &G.g_w,
+                         &ticks, &cu, &csingle};
+        (void)hipLaunchKernel(ts.kfn, dim3((unsigned)blocks), dim3(ts.threads), kargs, ts.lds, st);
+    }
+    const hipError_t le = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+        fprintf(stderr, "corrfunc_amd: graph capture failed: %s\n", hipGetErrorString(le != hipSuccess ? le : ce));
+        if (graph) (void)hipGraphDestroy(graph);
+        throw HipFailure();
+    }
+    G.graph = graph;
+    CF_CHECK(hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0));
+    G.ncells_cap.assign(ncat, ncap);
+    return EXIT_SUCCESS;
+}
+
+// Replays the graph and packs the results as cf_amd_countpairs_batch does.
+static int bg_launch(BatchGraph& G, std::vector<CallOut>& outs)
+{
+    Engine& E = eng();
+    const double t_start = now_s();
+    const int ncat = G.ncat, nslots = G.nbin;
+    const size_t tot = (size_t)ncat * nslots;
+    CF_CHECK(hipEventRecord(E.ev0, E.stream));
+    CF_CHECK(hipGraphLaunch(G.exec, E.stream));
+    CF_CHECK(hipEventRecord(E.ev1, E.stream));
+    std::vector<uint64_t> np(tot);
+    std::vector<double> avg(tot), wv(tot), ws(2 * (size_t)ncat), tk(ncat);
+    std::vector<BgCat> cats(ncat);
+    int fail = 0, ncp = 0, nitems = 0;
+    CF_CHECK(hipMemcpyAsync(np.data(), G.g_np, sizeof(uint64_t) * tot, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(avg.data(), G.g_avg, sizeof(double) * tot, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(wv.data(), G.g_w, sizeof(double) * tot, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(ws.data(), G.ws, sizeof(double) * 2 * ncat, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(tk.data(), G.tested, sizeof(double) * ncat, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(cats.data(), G.cats, sizeof(BgCat) * ncat, hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(&fail, G.fail, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(&ncp, G.cp_off + ncat, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipMemcpyAsync(&nitems, G.nitems_eff, sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    CF_CHECK(hipStreamSynchronize(E.stream));
+    if (fail) {
+        fprintf(stderr, "Error: the graph batch cannot count these positions (%s%s%s%s%s); create the graph again "
+                        "for them\n",
+                fail & kBgFailGrid ? "rmax >= half the box " : "", fail & kBgFailCells ? "more cells " : "",
+                fail & kBgFailOob ? "positions outside the grid " : "",
+                fail & kBgFailPairs ? "more cell pairs " : "", fail & kBgFailItems ? "more work items " : "");
+        return EXIT_FAILURE;
+    }
+    float gms = 0;
+    CF_CHECK(hipEventElapsedTime(&gms, E.ev0, E.ev1));
+    outs.assign(ncat, CallOut());
+    double tested_all = 0;
+    for (int k = 0; k < ncat; k++) {
+        CallOut& R = outs[k];
+        R.rupp = G.rupp;
+        R.nbin = G.nbin;
+        R.need_w = G.need_w;
+        R.np.assign(np.begin() + (size_t)k * nslots, np.begin() + (size_t)(k + 1) * nslots);
+        R.avg.assign(avg.begin() + (size_t)k * nslots, avg.begin() + (size_t)(k + 1) * nslots);
+        R.wavg.assign(wv.begin() + (size_t)k * nslots, wv.begin() + (size_t)(k + 1) * nslots);
+        R.wsum = ws[2 * k];
+        R.wsq = ws[2 * k + 1];
+        tested_all += tk[k];
+    }
+    cf_amd_stats st;
+    memset(&st, 0, sizeof st);
+    st.num_cell_pairs = ncp;
+    st.num_work_items = nitems;
+    st.tested_pairs = tested_all;
+    st.kernel_ms = gms;  // the whole replayed graph
+    st.total_ms = (now_s() - t_start) * 1e3;
+    st.engine = CF_AMD_ENGINE_TUNED;
+    st.hist_copies = G.hist_copies;
+    st.queue_fetch = G.queue_fetch;
+    st.sched = G.sched;
+    if (ncat > 0) {
+        st.nmesh[0] = cats[ncat - 1].g.nx; st.nmesh[1] = cats[ncat - 1].g.ny; st.nmesh[2] = cats[ncat - 1].g.nz;
+        for (int d = 0; d < 3; d++) st.refine[d] = cats[ncat - 1].refine[d];
+    }
     E.last = st;
     return EXIT_SUCCESS;
 }
@@ -2674,6 +3168,78 @@ int cf_amd_countpairs_batch(int ncat, const int64_t* ND, void** X, void** Y, voi
         }
         reset_bin_refine_factors(options);
         if (options->c_api_timer) options->c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+struct cf_amd_batch_graph {
+    cfamd::BatchGraph g;
+    struct config_options options;
+};
+
+int cf_amd_batch_graph_create(int ncat, const int64_t* ND, void** X, void** Y, void** Z, void** W,
+                              const int numthreads, const char* binfile, struct config_options* options,
+                              struct extra_options* extra, cf_amd_batch_graph** graph)
+{
+    (void)numthreads;
+    if (ncat <= 0 || !ND || !X || !Y || !Z || !graph) return EXIT_FAILURE;
+    if (check_common(options) != EXIT_SUCCESS) return EXIT_FAILURE;
+    *graph = nullptr;
+    return guarded([&]() -> int {
+        struct extra_options dummy;
+        if (extra == NULL) { dummy = get_extra_options(NONE); extra = &dummy; }
+        fix_maxcells(options);
+        fix_refine(options);
+        options->sort_on_z = 1;
+        std::unique_ptr<cf_amd_batch_graph> h(new cf_amd_batch_graph());
+        h->options = *options;
+        if (bg_create(ncat, ND, X, Y, Z, W, binfile, options, extra, h->g) != EXIT_SUCCESS) return EXIT_FAILURE;
+        reset_bin_refine_factors(options);
+        *graph = h.release();
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_batch_graph_launch(cf_amd_batch_graph* graph, results_countpairs* results)
+{
+    if (!graph || !results) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        const double t0 = now_s();
+        std::vector<CallOut> outs;
+        if (bg_launch(graph->g, outs) != EXIT_SUCCESS) return EXIT_FAILURE;
+        const int need_avg = graph->options.need_avg_sep;
+        for (int k = 0; k < graph->g.ncat; k++) {
+            CallOut& R = outs[k];
+            finalize(R, 1, graph->g.nd[k], need_avg, 1);
+            const int nbin = R.nbin;
+            results_countpairs* r = &results[k];
+            r->nbin = nbin;
+            r->npairs = (uint64_t*)malloc(sizeof(uint64_t) * nbin);
+            r->rupp = (double*)malloc(sizeof(double) * nbin);
+            r->rpavg = (double*)calloc(nbin, sizeof(double));
+            r->weightavg = (double*)calloc(nbin, sizeof(double));
+            if (!r->npairs || !r->rupp || !r->rpavg || !r->weightavg) {
+                for (int q = 0; q <= k; q++) free_results(&results[q]);
+                return EXIT_FAILURE;
+            }
+            for (int i = 0; i < nbin; i++) {
+                r->npairs[i] = R.np[i];
+                r->rupp[i] = R.rupp[i];
+                r->rpavg[i] = need_avg ? R.avg[i] : 0.0;
+                r->weightavg[i] = R.need_w ? R.wavg[i] : 0.0;
+            }
+        }
+        if (graph->options.c_api_timer) graph->options.c_api_time = now_s() - t0;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_batch_graph_destroy(cf_amd_batch_graph* graph)
+{
+    if (!graph) return EXIT_SUCCESS;
+    return guarded([&]() -> int {
+        CF_CHECK(hipStreamSynchronize(eng().stream));
+        delete graph;
         return EXIT_SUCCESS;
     });
 }

@@ -122,12 +122,10 @@ __device__ __forceinline__ int cell_and_slab(const GridArgs& g, double X, double
 }
 
 template <typename T>
-__global__ void k_bucket_count(int64_t n, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
-                               GridArgs g, int* __restrict__ bucket, int* __restrict__ counts,
-                               unsigned long long* __restrict__ oob_count)
+__device__ __forceinline__ void bucket_count_one(int64_t i, const T* __restrict__ X, const T* __restrict__ Y,
+                                                 const T* __restrict__ Z, const GridArgs& g, int* __restrict__ bucket,
+                                                 int* __restrict__ counts, unsigned long long* __restrict__ oob_count)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
     int c, oob;
     const int s = cell_and_slab(g, (double)X[i], (double)Y[i], (double)Z[i], &c, &oob);
     if (oob) {
@@ -138,6 +136,15 @@ __global__ void k_bucket_count(int64_t n, const T* __restrict__ X, const T* __re
     const int b = c * kSlabs + s;
     bucket[i] = b;
     atomicAdd(&counts[b], 1);
+}
+
+template <typename T>
+__global__ void k_bucket_count(int64_t n, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
+                               GridArgs g, int* __restrict__ bucket, int* __restrict__ counts,
+                               unsigned long long* __restrict__ oob_count)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) bucket_count_one(i, X, Y, Z, g, bucket, counts, oob_count);
 }
 
 // Exclusive scan over int32 counts, three phases, 1024 items per block.
@@ -191,14 +198,13 @@ __global__ void k_scan_add(int64_t n, int* __restrict__ out, const int* __restri
 }
 
 template <typename T>
-__global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
-                          const T* __restrict__ W, GridArgs g, const int* __restrict__ bucket,
-                          const int* __restrict__ bstart, int* __restrict__ fill,
-                          double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
-                          double* __restrict__ w, double4* __restrict__ d4, uint2* __restrict__ q)
+__device__ __forceinline__ void scatter_one(int64_t i, const T* __restrict__ X, const T* __restrict__ Y,
+                                            const T* __restrict__ Z, const T* __restrict__ W, const GridArgs& g,
+                                            const int* __restrict__ bucket, const int* __restrict__ bstart,
+                                            int* __restrict__ fill, double* __restrict__ x, double* __restrict__ y,
+                                            double* __restrict__ z, double* __restrict__ w, double4* __restrict__ d4,
+                                            uint2* __restrict__ q)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
     const int b = bucket[i];
     if (b < 0) return;  // outside the grid: counted by k_bucket_count, reported by the host
     const int pos = bstart[b] + atomicAdd(&fill[b], 1);
@@ -215,15 +221,25 @@ __global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restric
     q[pos] = quantize_pos(xv - ox, yv - oy, zv - oz, g.qinv);
 }
 
+template <typename T>
+__global__ void k_scatter(int64_t n, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
+                          const T* __restrict__ W, GridArgs g, const int* __restrict__ bucket,
+                          const int* __restrict__ bstart, int* __restrict__ fill,
+                          double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+                          double* __restrict__ w, double4* __restrict__ d4, uint2* __restrict__ q)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) scatter_one(i, X, Y, Z, W, g, bucket, bstart, fill, x, y, z, w, d4, q);
+}
+
 // Per-cell bounding boxes, one wave per cell (lanes stride over the cell's
 // particles, then a shuffle reduction; min/max are exact, so the order of the
 // reduction does not matter). Empty cells get (+DBL_MAX, -DBL_MAX).
-__global__ void k_cell_bounds(int64_t ncells, const int* __restrict__ bstart, const double* __restrict__ x,
-                              const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ bounds)
+__device__ __forceinline__ void cell_bounds_one(int64_t c, const int* __restrict__ bstart,
+                                                const double* __restrict__ x, const double* __restrict__ y,
+                                                const double* __restrict__ z, double* __restrict__ bounds)
 {
-    const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (c >= ncells) return;  // uniform per wave
     double b[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
     const int e = bstart[(c + 1) * kSlabs];
     for (int k = bstart[c * kSlabs] + lane; k < e; k += 64) {
@@ -244,6 +260,13 @@ __global__ void k_cell_bounds(int64_t ncells, const int* __restrict__ bstart, co
     }
     if (lane == 0)
         for (int d = 0; d < 6; d++) bounds[c * 6 + d] = b[d];
+}
+
+__global__ void k_cell_bounds(int64_t ncells, const int* __restrict__ bstart, const double* __restrict__ x,
+                              const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ bounds)
+{
+    const int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    if (c < ncells) cell_bounds_one(c, bstart, x, y, z, bounds);  // uniform per wave
 }
 
 __global__ void k_weight_sums(int64_t n, const double* __restrict__ w, double* __restrict__ out)
@@ -1435,13 +1458,19 @@ __device__ __forceinline__ float sane_cost(float c)
 // holds cell pair order[t] (order == nullptr: t). With a cost array (log2
 // units), a chunk whose predicted cost exceeds total / cap_div is split into
 // ceil(cost / cap) parts (at most kMaxParts).
+// ncp_dev (optional): a device count that bounds ncp; entries of the count
+// array past it are 0 (a graph-captured batch scans its whole capacity).
 __global__ void k_count_chunks(int ncp, const int* __restrict__ order, const CellPairRec* __restrict__ cps,
                                const DevLattice* __restrict__ lats, int batch, const float* __restrict__ cost,
                                const double* __restrict__ total, double cap_div, int* __restrict__ nparts,
-                               int* __restrict__ nitems)
+                               int* __restrict__ nitems, const int* __restrict__ ncp_dev = nullptr)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncp) return;
+    if (ncp_dev && t >= *ncp_dev) {
+        nitems[t] = 0;
+        return;
+    }
     const int c = order ? order[t] : t;
     const int* bs1 = lats[batch ? cps[c].pad : 0].bstart;
     const int n = (int)cell_count(bs1, cps[c].c1);
@@ -1456,13 +1485,15 @@ __global__ void k_count_chunks(int ncp, const int* __restrict__ order, const Cel
     nitems[t] = nch * parts;
 }
 
+// Items are written only below items_cap (a graph-captured batch's capacity).
 __global__ void k_emit_chunks(int ncp, const int* __restrict__ order, const CellPairRec* __restrict__ cps,
                               const DevLattice* __restrict__ lats, int batch, PairParams P,
                               const int* __restrict__ off, const int* __restrict__ nparts,
-                              WorkItem* __restrict__ items)
+                              WorkItem* __restrict__ items, const int* __restrict__ ncp_dev = nullptr,
+                              int items_cap = INT_MAX)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncp) return;
+    if (t >= ncp || (ncp_dev && t >= *ncp_dev)) return;
     const int c = order ? order[t] : t;
     const CellPairRec cp = cps[c];
     const int* bs1 = lats[batch ? cp.pad : 0].bstart;
@@ -1489,7 +1520,8 @@ __global__ void k_emit_chunks(int ncp, const int* __restrict__ order, const Cell
         w.nparts = parts;
         for (int p = 0; p < parts; p++) {
             w.part = p;
-            items[o++] = w;
+            if (o < items_cap) items[o] = w;
+            o++;
         }
     }
 }

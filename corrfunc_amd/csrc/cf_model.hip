@@ -44,10 +44,13 @@ __device__ __forceinline__ void mlp_hidden(const float* __restrict__ W, const fl
 }
 
 // Predicted log2 cost of n samples (X row-major [n][F]).
+// n_dev (optional): a device count that bounds n (graph-captured batches).
 __global__ __launch_bounds__(256) void k_mlp_predict(int n, const float* __restrict__ X,
                                                      const float* __restrict__ norm,
-                                                     const float* __restrict__ params, float* __restrict__ out)
+                                                     const float* __restrict__ params, float* __restrict__ out,
+                                                     const int* __restrict__ n_dev = nullptr)
 {
+    if (n_dev) n = min(n, *n_dev);
     __shared__ float W[kNumParams];
     __shared__ float nm[kNumNorm];
     for (int p = threadIdx.x; p < kNumParams; p += blockDim.x) W[p] = params[p];

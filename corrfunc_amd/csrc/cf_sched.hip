@@ -13,14 +13,18 @@
 //   4. splits heavy cell pairs into parts of their secondary window
 //      (k_count_chunks / k_emit_chunks with the cost array).
 // The counts do not depend on the order or the split: every (i, j) pair is
-// still tested exactly once.
+// still tested exactly once. Every kernel takes an optional device count
+// (`ncp_dev`) that bounds `ncp`: a graph-captured batch launches them over its
+// capacity and the count its earlier steps wrote.
 #include "cf_device.h"
 
 namespace cfamd {
 
 __global__ void k_cp_features(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
-                              int batch, PairParams P, float* __restrict__ feats)
+                              int batch, PairParams P, float* __restrict__ feats,
+                              const int* __restrict__ ncp_dev = nullptr)
 {
+    if (ncp_dev) ncp = min(ncp, *ncp_dev);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncp) return;
     const CellPairRec cp = cps[t];
@@ -58,8 +62,9 @@ __global__ void k_cp_features(int ncp, const CellPairRec* __restrict__ cps, cons
 // The plain heuristic: log2 of the number of pairs tested, N1*N2 (N1(N1-1)/2
 // for a cell with itself).
 __global__ void k_cost_heuristic(int ncp, const CellPairRec* __restrict__ cps, const DevLattice* __restrict__ lats,
-                                 int batch, float* __restrict__ cost)
+                                 int batch, float* __restrict__ cost, const int* __restrict__ ncp_dev = nullptr)
 {
+    if (ncp_dev) ncp = min(ncp, *ncp_dev);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncp) return;
     const CellPairRec cp = cps[t];
@@ -81,8 +86,10 @@ __device__ __forceinline__ int sort_bucket(float c, float light_below)
 }
 
 // total = sum over cell pairs of 2^cost.
-__global__ void k_cost_total(int ncp, const float* __restrict__ cost, double* __restrict__ total)
+__global__ void k_cost_total(int ncp, const float* __restrict__ cost, double* __restrict__ total,
+                             const int* __restrict__ ncp_dev = nullptr)
 {
+    if (ncp_dev) ncp = min(ncp, *ncp_dev);
     double v = 0.0;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ncp; t += gridDim.x * blockDim.x)
         v += exp2((double)sane_cost(cost[t]));
@@ -92,15 +99,16 @@ __global__ void k_cost_total(int ncp, const float* __restrict__ cost, double* __
 
 __device__ __forceinline__ float light_threshold(int ncp, const double* total, float heavy)
 {
-    return (float)log2(fmax(*total / (double)ncp * (double)heavy, 1.0));
+    return (float)log2(fmax(*total / (double)(ncp > 0 ? ncp : 1) * (double)heavy, 1.0));
 }
 
 // Per-block bucket histogram (LDS atomics), written bucket-major:
 // counts[b * nblocks + block]. kSortBlock elements per block.
 __global__ __launch_bounds__(256) void k_cost_hist(int ncp, const float* __restrict__ cost,
                                                    const double* __restrict__ total, float heavy, int nblocks,
-                                                   int* __restrict__ counts)
+                                                   int* __restrict__ counts, const int* __restrict__ ncp_dev = nullptr)
 {
+    if (ncp_dev) ncp = min(ncp, *ncp_dev);
     __shared__ int h[kSortBuckets];
     for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x) h[b] = 0;
     __syncthreads();
@@ -116,8 +124,10 @@ __global__ __launch_bounds__(256) void k_cost_hist(int ncp, const float* __restr
 // order[pos] = cell pair, pos = offs[b * nblocks + block] + rank within the block.
 __global__ __launch_bounds__(256) void k_cost_place(int ncp, const float* __restrict__ cost,
                                                     const double* __restrict__ total, float heavy, int nblocks,
-                                                    const int* __restrict__ offs, int* __restrict__ order)
+                                                    const int* __restrict__ offs, int* __restrict__ order,
+                                                    const int* __restrict__ ncp_dev = nullptr)
 {
+    if (ncp_dev) ncp = min(ncp, *ncp_dev);
     __shared__ int cursor[kSortBuckets];
     for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x)
         cursor[b] = offs[(int64_t)b * nblocks + blockIdx.x];

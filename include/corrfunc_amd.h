@@ -131,6 +131,27 @@ int cf_amd_countpairs_batch(int ncat, const int64_t *ND, void **X, void **Y, voi
                             const int numthreads, const char *binfile, results_countpairs *results,
                             struct config_options *options, struct extra_options *extra);
 
+/* The batched DD(r) of cf_amd_countpairs_batch captured once into a HIP graph
+ * and replayed on new positions of the same catalog sizes (jackknife and
+ * bootstrap resamples written into the same device arrays). No counterpart in
+ * the reference. Every decision the batch takes from the positions -- each
+ * catalog's extent and grid, its cell pairs and the work list -- is taken on the
+ * device, so a launch counts whatever the arrays hold when it runs; results[k]
+ * equals countpairs() on catalog k then (release each with free_results).
+ * X[k], Y[k], Z[k] (and W[k] with PAIR_PRODUCT weights) must be device memory
+ * that stays valid while the graph exists. The box must be periodic with its
+ * size given on every axis. Capacities are fixed at create: cells from the box,
+ * cell pairs and work items from the capture-time positions plus 25 %; a launch
+ * whose positions need more fails with a message (create the graph again). The
+ * work list is ordered and split by the scheduler selected at create (the cost
+ * model's parameters are read at each launch). */
+typedef struct cf_amd_batch_graph cf_amd_batch_graph;
+int cf_amd_batch_graph_create(int ncat, const int64_t *ND, void **X, void **Y, void **Z, void **W,
+                              const int numthreads, const char *binfile, struct config_options *options,
+                              struct extra_options *extra, cf_amd_batch_graph **graph);
+int cf_amd_batch_graph_launch(cf_amd_batch_graph *graph, results_countpairs *results);
+int cf_amd_batch_graph_destroy(cf_amd_batch_graph *graph);
+
 /* countpairs() for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc). It has
  * no counterpart in the reference. Same arguments and results as countpairs()
