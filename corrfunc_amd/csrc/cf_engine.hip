@@ -9,6 +9,7 @@
 
 #include <ctype.h>
 #include <math.h>
+#include <signal.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -201,6 +202,58 @@ static double now_s()
     struct timeval t;
     gettimeofday(&t, NULL);
     return t.tv_sec + 1e-6 * t.tv_usec;
+}
+
+// --------------------------------------------------------- interrupts
+// As the reference (utils/macros.h:145-167, theory/DD/countpairs_impl.c.src:
+// 31-37, 191-193, 554-566): while a counting call runs, SIGTERM, SIGINT and
+// SIGHUP are caught; the call then stops at its next phase boundary (after
+// gridding, after the cell pairs, after the pair kernel) and returns
+// EXIT_FAILURE. Unlike the reference, the previous handlers are restored on
+// every return and the next call starts uninterrupted. The outermost scope of
+// nested engine calls (the autotuner's candidate runs) installs the handlers.
+static volatile sig_atomic_t g_interrupt_signo = 0;
+static int g_interrupt_depth = 0;
+
+static void on_interrupt(int signo) { g_interrupt_signo = signo; }
+
+struct InterruptScope {
+    static constexpr int kSignals[3] = {SIGTERM, SIGINT, SIGHUP};
+    void (*prev[3])(int) = {SIG_ERR, SIG_ERR, SIG_ERR};
+    bool outer;
+    InterruptScope() : outer(g_interrupt_depth++ == 0)
+    {
+        if (!outer) return;
+        g_interrupt_signo = 0;
+        for (int i = 0; i < 3; i++) {
+            prev[i] = signal(kSignals[i], on_interrupt);
+            if (prev[i] == SIG_ERR) fprintf(stderr, "Can not handle signal = %d\n", kSignals[i]);
+        }
+    }
+    ~InterruptScope()
+    {
+        g_interrupt_depth--;
+        if (!outer) return;
+        for (int i = 0; i < 3; i++)
+            if (prev[i] != SIG_ERR && signal(kSignals[i], prev[i]) == SIG_ERR)
+                fprintf(stderr, "Could not reset signal handler for signal = %d\n", kSignals[i]);
+    }
+};
+
+// Throws InputError (the entry point returns EXIT_FAILURE) after a caught signal.
+// CORRFUNC_AMD_TEST_SIGNAL=<signo> raises that signal at the first check, to
+// exercise the path from a test.
+static void check_interrupt()
+{
+    if (const char* ts = getenv("CORRFUNC_AMD_TEST_SIGNAL"))
+        if (!g_interrupt_signo && atoi(ts) > 0) raise(atoi(ts));
+    if (g_interrupt_signo) {
+        // nothing queued may still write into buffers the unwinding frees
+        (void)hipStreamSynchronize(eng().stream);
+        fprintf(stderr, "Received signal = `%s' (signo = %d). Aborting \n", strsignal(g_interrupt_signo),
+                (int)g_interrupt_signo);
+        throw InputError();
+    }
 }
 
 // Clears device words with a fill kernel on the engine's stream (no memset
@@ -1170,6 +1223,13 @@ static int autotune_choice(const CallSpec& c, const struct config_options* o, st
         E.last_autotuned = std::max(E.last_autotuned, 1);
         return EXIT_SUCCESS;
     }
+    // one handler scope over all candidate runs; the forced settings are
+    // cleared however the measuring ends (an error or an interrupt throws)
+    InterruptScope interrupts;
+    struct TuneReset {
+        Engine& E;
+        ~TuneReset() { E.force_fetch = 0; E.force_R = 0; E.tuning = false; }
+    } reset{E};
     E.tuning = true;
     double best = 1e300;
     std::array<int, 5> pick = {2, 2, 1, 0, 1};
@@ -1218,9 +1278,6 @@ static int autotune_choice(const CallSpec& c, const struct config_options* o, st
             }
         }
     }
-    E.force_fetch = 0;
-    E.force_R = 0;
-    E.tuning = false;
     E.tune_cache[key] = pick;
     tune_cache_store(key, pick);
     out = pick;
@@ -1231,6 +1288,7 @@ static int autotune_choice(const CallSpec& c, const struct config_options* o, st
 static int run_core(const CallSpec& c, struct config_options* o, struct extra_options* extra, CallOut& R)
 {
     Engine& E = eng();
+    InterruptScope interrupts;
     cf_amd_stats st;
     memset(&st, 0, sizeof st);
     const double t_start = now_s();
@@ -1393,6 +1451,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         L2 = L1;
     }
     const double t_grid = now_s();
+    check_interrupt();
 
     // ---- cell pairs
     double max3 = -1, max2 = -1, max1 = -1;
@@ -1406,6 +1465,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     if (cellpairs_device(L1, L2, refine, per, wrap, max3, max2, max1, o->enable_min_sep_opt, c.autocorr, &cps, &ncp,
                          &tested) != EXIT_SUCCESS)
         return EXIT_FAILURE;
+    check_interrupt();
 
     // ---- launch parameters
     PairParams P;
@@ -1501,6 +1561,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
         CF_CHECK(hipMemcpyAsync(R.timings.data(), rows, sizeof(TimingRow) * ncp, hipMemcpyDeviceToHost, E.stream));
     }
     CF_CHECK(hipStreamSynchronize(E.stream));
+    check_interrupt();
     float kms = 0, lms = 0;
     CF_CHECK(hipEventElapsedTime(&kms, E.ev0, E.ev1));
     CF_CHECK(hipEventElapsedTime(&lms, E.ev_list, E.ev0));
@@ -1536,6 +1597,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
                      struct config_options* options, struct extra_options* extra, std::vector<CallOut>& outs)
 {
     Engine& E = eng();
+    InterruptScope interrupts;
     const double t_start = now_s();
     const size_t esize = options->float_type;
     const int need_w = extra->weight_method != NONE;
@@ -1673,6 +1735,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         }
         ncp_all += ncps[k];
     }
+    check_interrupt();
     coord_scale = 6 * coord_scale + wrap0[0] + wrap0[1] + wrap0[2] + rmax;
     // concatenated work list, emitted in place
     CellPairRec* cps = (CellPairRec*)E.arena.get("bcp_all", sizeof(CellPairRec) * ((size_t)ncp_all + 1));
@@ -1749,6 +1812,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
         CF_CHECK(hipMemcpyAsync(wsums.data(), ws, sizeof(double) * 2 * ncat, hipMemcpyDeviceToHost, E.stream));
     }
     CF_CHECK(hipStreamSynchronize(E.stream));
+    check_interrupt();
     float kms = 0, lms = 0;
     CF_CHECK(hipEventElapsedTime(&kms, E.ev0, E.ev1));
     CF_CHECK(hipEventElapsedTime(&lms, E.ev_list, E.ev0));
@@ -2182,6 +2246,7 @@ static int bg_create(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
 static int bg_launch(BatchGraph& G, std::vector<CallOut>& outs)
 {
     Engine& E = eng();
+    InterruptScope interrupts;
     const double t_start = now_s();
     const int ncat = G.ncat, nslots = G.nbin;
     const size_t tot = (size_t)ncat * nslots;
@@ -2202,6 +2267,7 @@ static int bg_launch(BatchGraph& G, std::vector<CallOut>& outs)
     CF_CHECK(hipMemcpyAsync(&ncp, G.cp_off + ncat, sizeof(int), hipMemcpyDeviceToHost, E.stream));
     CF_CHECK(hipMemcpyAsync(&nitems, G.nitems_eff, sizeof(int), hipMemcpyDeviceToHost, E.stream));
     CF_CHECK(hipStreamSynchronize(E.stream));
+    check_interrupt();
     if (fail) {
         fprintf(stderr, "Error: the graph batch cannot count these positions (%s%s%s%s%s); create the graph again "
                         "for them\n",
@@ -2356,8 +2422,8 @@ static int model_train_dev(const float* dX, const float* dY, int64_t n, int epoc
         for (int64_t b0 = 0; b0 < n; b0 += batch) {
             const int nb = (int)std::min<int64_t>(batch, n - b0);
             model_grad_dev(nb, d_idx + b0, dX, dY, M.d_norm, M.d_params, 1.f / nb, d_grad, d_loss);
-            k_adam<<<(kNumParams + 255) / 256, 256, 0, E.stream>>>(M.d_params, d_grad, d_m, d_v, ++step, lr, 0.9f,
-                                                                  0.999f, 1e-8f);
+            k_adam<<<(kNumParams + 255) / 256, 256, 0, E.stream>>>(M.d_params, d_grad, d_m, d_v,
+                                                                  adam_consts(++step, lr));
             CF_CHECK(hipGetLastError());
         }
         CF_CHECK(hipStreamSynchronize(E.stream));  // perm is rewritten next epoch
@@ -3422,6 +3488,28 @@ int cf_amd_model_grad(const float* X, const float* y, int64_t n, const float* pa
         CF_CHECK(hipMemcpyAsync(&l, dL, sizeof(float), hipMemcpyDeviceToHost, E.stream));
         CF_CHECK(hipStreamSynchronize(E.stream));
         if (loss) *loss = l;
+        return EXIT_SUCCESS;
+    });
+}
+
+int cf_amd_model_adam_step(float* params, const float* grad, float* m, float* v, int step, float lr)
+{
+    if (!params || !grad || !m || !v || step < 1) return EXIT_FAILURE;
+    return guarded([&]() -> int {
+        Engine& E = eng();
+        float* d = (float*)E.arena.get("adam_check", sizeof(float) * 4 * kNumParams);
+        const size_t b = sizeof(float) * kNumParams;
+        CF_CHECK(hipMemcpyAsync(d, params, b, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipMemcpyAsync(d + kNumParams, grad, b, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipMemcpyAsync(d + 2 * kNumParams, m, b, hipMemcpyHostToDevice, E.stream));
+        CF_CHECK(hipMemcpyAsync(d + 3 * kNumParams, v, b, hipMemcpyHostToDevice, E.stream));
+        k_adam<<<(kNumParams + 255) / 256, 256, 0, E.stream>>>(d, d + kNumParams, d + 2 * kNumParams,
+                                                                d + 3 * kNumParams, adam_consts(step, lr));
+        CF_CHECK(hipGetLastError());
+        CF_CHECK(hipMemcpyAsync(params, d, b, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipMemcpyAsync(m, d + 2 * kNumParams, b, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipMemcpyAsync(v, d + 3 * kNumParams, b, hipMemcpyDeviceToHost, E.stream));
+        CF_CHECK(hipStreamSynchronize(E.stream));
         return EXIT_SUCCESS;
     });
 }

@@ -164,20 +164,33 @@ __global__ void k_mlp_reduce(int nblocks, const float* __restrict__ partial, con
     }
 }
 
-// One Adam step (Kingma & Ba 2015, bias-corrected moments), step >= 1.
+// One Adam step (Kingma & Ba 2015, bias-corrected moments). The constants come
+// from the host, computed in FP64 and rounded once: 1 - beta1, 1 - beta2 and
+// the bias corrections 1 / (1 - beta^step) differ from the same quantities
+// formed in FP32 from rounded betas (1 - 0.999f is 1.3e-5 off 0.001).
+struct AdamConsts {
+    float lr, b1, omb1, b2, omb2, inv_bc1, inv_bc2, eps;
+};
+
+static inline AdamConsts adam_consts(int step, float lr, double beta1 = 0.9, double beta2 = 0.999, double eps = 1e-8)
+{
+    return AdamConsts{lr, (float)beta1, (float)(1.0 - beta1), (float)beta2, (float)(1.0 - beta2),
+                      (float)(1.0 / (1.0 - pow(beta1, step))), (float)(1.0 / (1.0 - pow(beta2, step))), (float)eps};
+}
+
 __global__ void k_adam(float* __restrict__ params, const float* __restrict__ grad, float* __restrict__ m,
-                       float* __restrict__ v, int step, float lr, float beta1, float beta2, float eps)
+                       float* __restrict__ v, AdamConsts c)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kNumParams) return;
     const float g = grad[p];
-    const float mp = beta1 * m[p] + (1.f - beta1) * g;
-    const float vp = beta2 * v[p] + (1.f - beta2) * g * g;
+    const float mp = c.b1 * m[p] + c.omb1 * g;
+    const float vp = c.b2 * v[p] + c.omb2 * (g * g);
     m[p] = mp;
     v[p] = vp;
-    const float mh = mp / (1.f - powf(beta1, (float)step));
-    const float vh = vp / (1.f - powf(beta2, (float)step));
-    params[p] -= lr * mh / (sqrtf(vh) + eps);
+    const float mh = mp * c.inv_bc1;
+    const float vh = vp * c.inv_bc2;
+    params[p] -= c.lr * mh / (sqrtf(vh) + c.eps);
 }
 
 }  // namespace cfamd

@@ -158,6 +158,21 @@ def grad(X, y, p, norm):
     return float(loss[0]), g
 
 
+def adam_step(p, g, m, v, step, lr):
+    """One Adam step of the GPU training kernel on host arrays; returns (p, m, v)."""
+    p = np.array(p, np.float32)
+    g = np.ascontiguousarray(g, np.float32)
+    m = np.array(m, np.float32)
+    v = np.array(v, np.float32)
+    L = _lib.lib()
+    fp = C.POINTER(C.c_float)
+    L.cf_amd_model_adam_step.argtypes = [fp, fp, fp, fp, C.c_int, C.c_float]
+    L.cf_amd_model_adam_step.restype = C.c_int
+    if L.cf_amd_model_adam_step(_fp(p), _fp(g), _fp(m), _fp(v), int(step), float(lr)) != 0:
+        raise RuntimeError("cf_amd_model_adam_step failed")
+    return p, m, v
+
+
 # ------------------------------------------------------------- evaluation
 def heuristic_log2_cost(X):
     """The N1*N2 heuristic from the features: log2(1 + N1*N2), or
@@ -181,15 +196,59 @@ def list_schedule_makespan(true_cost, order, workers):
     return max(free)
 
 
+def engine_items(X, est, workers, heavy=4.0, split=4, max_parts=32):
+    """The work items the engine makes from cell pairs whose log2 cost estimate
+    is `est` (cf_sched.hip, k_count_chunks): the order (cell pairs with an
+    estimate >= heavy x the mean go first, heaviest first in quarter-octave
+    buckets; the rest keep list order), and per cell pair its 64-primary chunks,
+    each split into ceil(chunk estimate / cap) parts (at most max_parts) when
+    above cap = total estimate / (split x workers).
+
+    Returns (cell pair index, number of items) per cell pair in launch order."""
+    X = np.asarray(X, np.float64)
+    est = np.asarray(est, np.float64)
+    n = len(est)
+    c = np.exp2(est)
+    total = c.sum()
+    light = np.log2(max(total / max(n, 1) * heavy, 1.0))
+    hv = est >= light
+    bucket = np.floor((48.0 - est[hv]) * 4.0)
+    hidx = np.nonzero(hv)[0]
+    order = np.concatenate([hidx[np.argsort(bucket, kind="stable")], np.nonzero(~hv)[0]])
+    nch = np.maximum(1.0, np.round(np.exp2(X[:, 2])))
+    cap = total / (split * workers)
+    per_chunk = c / nch
+    parts = np.where(per_chunk > cap, np.minimum(max_parts, np.ceil(per_chunk / cap)), 1.0)
+    return order, (nch * parts)[order].astype(np.int64)
+
+
+def item_makespan(true_cost, order, nitems, workers):
+    """Makespan of greedy list scheduling of the items of `engine_items` (each
+    cell pair's true cost spread evenly over its items)."""
+    import heapq
+    free = [0.0] * workers
+    heapq.heapify(free)
+    for k, m in zip(order, nitems):
+        each = true_cost[k] / m
+        for _ in range(int(m)):
+            t = heapq.heappop(free)
+            heapq.heappush(free, t + each)
+    return max(free)
+
+
 def evaluate(X, y, pred=None, offset_from=None, workers=2048):
     """Prediction error and load imbalance of the model and of the heuristic on
     recorded samples (X, y = log2 ticks).
 
     rmse_*: RMS error in log2 units; the heuristic gets its best constant
     offset (fitted on `offset_from` = (X, y) when given, else on (X, y)).
-    imbalance_*: makespan / (total / workers) - 1 of greedy list scheduling in
-    the order each cost estimate gives (descending), measured with the true
-    costs; "none" keeps the recorded order."""
+    imbalance_*: makespan / (total / workers) - 1 of greedy list scheduling of
+    whole cell pairs in the order each cost estimate gives (descending),
+    measured with the true costs; "none" keeps the recorded order.
+    items_imbalance_*: the same for the work items the engine makes with each
+    estimate (its heavy-first order and splitting, `engine_items`); "none" is
+    the list order with 64-primary chunks and no splitting, as the engine runs
+    without a scheduler."""
     X = np.asarray(X, np.float64)
     y = np.asarray(y, np.float64)
     h = heuristic_log2_cost(X)
@@ -210,4 +269,12 @@ def evaluate(X, y, pred=None, offset_from=None, workers=2048):
         out["rmse_learned"] = float(np.sqrt(np.mean((pred - y) ** 2)))
         out["imbalance_learned"] = (list_schedule_makespan(true, np.argsort(-pred, kind="stable"), workers)
                                     / ideal - 1)
+    nch = np.maximum(1, np.round(np.exp2(X[:, 2]))).astype(np.int64)
+    out["items_imbalance_none"] = item_makespan(true, np.arange(len(y)), nch, workers) / ideal - 1
+    ests = {"heuristic": h, "oracle_order": y}
+    if pred is not None:
+        ests["learned"] = pred
+    for name, est in ests.items():
+        order, nitems = engine_items(X, est, workers)
+        out["items_imbalance_" + name] = item_makespan(true, order, nitems, workers) / ideal - 1
     return out

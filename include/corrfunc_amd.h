@@ -109,6 +109,9 @@ int cf_amd_model_predict(const float *X, int64_t n, float *out);  /* log2 cost o
  * gradient with respect to `params` (the GPU's forward and backward kernels). */
 int cf_amd_model_grad(const float *X, const float *y, int64_t n, const float *params, const float *norm,
                       float *grad, float *loss);
+/* One Adam step of the training kernel (beta1 0.9, beta2 0.999, eps 1e-8,
+ * bias-corrected, step >= 1) on CF_AMD_MODEL_PARAMS host values, in place. */
+int cf_amd_model_adam_step(float *params, const float *grad, float *m, float *v, int step, float lr);
 
 /* Selects the pair kernel for subsequent calls; returns EXIT_SUCCESS/EXIT_FAILURE. */
 int cf_amd_set_engine(int kind);

@@ -65,6 +65,26 @@ def test_adam_first_step_moves_by_lr_times_sign():
     assert m == pytest.approx(0.1 * g) and v == pytest.approx(0.001 * g * g)
 
 
+def test_adam_second_step_by_hand():
+    """Moments and bias correction of step 2, worked by hand: g = 2 then -1."""
+    lr = 0.1
+    p1, m1, v1 = cm.adam_step(np.zeros(1), np.array([2.0]), np.zeros(1), np.zeros(1), 1, lr)
+    p2, m2, v2 = cm.adam_step(p1, np.array([-1.0]), m1, v1, 2, lr)
+    # m1 = 0.2, v1 = 0.004, m2 = 0.18 - 0.1 = 0.08, v2 = 0.003996 + 0.001 = 0.004996
+    assert m2[0] == pytest.approx(0.08, rel=1e-12) and v2[0] == pytest.approx(0.004996, rel=1e-12)
+    mh, vh = 0.08 / (1 - 0.81), 0.004996 / (1 - 0.998001)
+    assert p2[0] == pytest.approx(p1[0] - lr * mh / (np.sqrt(vh) + 1e-8), rel=1e-12)
+
+
+def test_adam_minimises_a_quadratic():
+    """Adam on f(p) = |p - c|^2 / 2 (gradient p - c) ends near c."""
+    c = np.array([1.5, -2.0, 0.25])
+    p, m, v = np.zeros(3), np.zeros(3), np.zeros(3)
+    for step in range(1, 3001):
+        p, m, v = cm.adam_step(p, p - c, m, v, step, 0.01)
+    assert np.abs(p - c).max() < 1e-3
+
+
 def test_make_norm_standardises():
     p, norm, X, y = _problem(200, 5)
     xs = (X - norm[:cm.F]) * norm[cm.F:2 * cm.F]
@@ -109,3 +129,37 @@ def test_evaluate_prefers_the_better_estimate():
     assert ev["rmse_learned"] < 0.1 < ev["rmse_heuristic"]
     assert ev["imbalance_learned"] <= ev["imbalance_heuristic"] + 1e-9
     assert ev["imbalance_oracle_order"] <= ev["imbalance_none"]
+
+
+def test_engine_items_order_and_splits_by_hand():
+    """One heavy cell pair (1000 of a total 1007, 4x the mean is 503.5) goes
+    first and is split into ceil(1000 / (1007 / (4 x 2 workers))) = 8 items;
+    the light ones keep list order, one item each (one 64-primary chunk)."""
+    X = np.zeros((8, scheduler.FEATURES))
+    est = np.log2([1.0, 1.0, 1000.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    order, nitems = scheduler.engine_items(X, est, workers=2)
+    assert list(order) == [2, 0, 1, 3, 4, 5, 6, 7]
+    assert list(nitems) == [8, 1, 1, 1, 1, 1, 1, 1]
+    # 8 items of 125 alternate over the 2 workers (500 each), then 7 of 1
+    true = np.exp2(est)
+    assert scheduler.item_makespan(true, order, nitems, 2) == 504.0
+    # a cell pair of 3 chunks splits each chunk: 3 x ceil(333.3 / 125.9) = 9 items
+    X[2, 2] = np.log2(3)
+    _, nitems = scheduler.engine_items(X, est, workers=2)
+    assert nitems[0] == 9
+
+
+def test_item_level_imbalance_prefers_the_better_estimate():
+    rng = np.random.default_rng(2)
+    n = 4000
+    X = np.zeros((n, scheduler.FEATURES))
+    n1 = rng.integers(1, 300, n)
+    n2 = rng.integers(1, 300, n)
+    X[:, 0], X[:, 1] = np.log2(1 + n1), np.log2(1 + n2)
+    X[:, 2] = np.log2(np.maximum(1, np.ceil(n1 / 64)))
+    # a few cell pairs far more expensive than N1*N2 says (what the model can learn)
+    f = np.where(rng.uniform(size=n) < 0.01, 200.0, 1.0)
+    y = np.log2(n1 * n2 * f)
+    ev = scheduler.evaluate(X, y, pred=y + rng.normal(0, 0.05, n), workers=256)
+    assert ev["items_imbalance_oracle_order"] < ev["items_imbalance_none"]
+    assert ev["items_imbalance_learned"] < ev["items_imbalance_heuristic"]

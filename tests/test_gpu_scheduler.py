@@ -63,6 +63,36 @@ def test_model_predict_matches_oracle(engine):
     assert np.array_equal(q, p) and np.array_equal(nm, norm)
 
 
+@pytest.mark.parametrize("step", [1, 2, 37])
+def test_adam_step_matches_oracle(engine, step):
+    """One k_adam update (the training kernel's) against the FP64 oracle's
+    adam_step, FP32 tolerance."""
+    rng = np.random.default_rng(step)
+    p = rng.normal(0, 0.5, cm.NPARAM).astype(np.float32)
+    g = (rng.normal(0, 1, cm.NPARAM) * np.exp(rng.uniform(-6, 2, cm.NPARAM))).astype(np.float32)
+    m = (rng.normal(0, 0.1, cm.NPARAM) if step > 1 else np.zeros(cm.NPARAM)).astype(np.float32)
+    v = (rng.uniform(0, 0.1, cm.NPARAM) if step > 1 else np.zeros(cm.NPARAM)).astype(np.float32)
+    gp, gm, gv = scheduler.adam_step(p, g, m, v, step, 3e-3)
+    p64, g64, m64, v64 = (a.astype(np.float64) for a in (p, g, m, v))
+    wp, wm, wv = cm.adam_step(p64, g64, m64, v64, step, 3e-3)
+    # moments: each constant and each product rounded once to FP32, then the sum
+    # (the terms may cancel, so the bound is on the terms, not on the result)
+    u = 2.0 ** -24
+    assert np.all(np.abs(gm - wm) <= 4 * u * (0.9 * np.abs(m64) + 0.1 * np.abs(g64)) + 1e-30)
+    assert np.all(np.abs(gv - wv) <= 5 * u * (0.999 * v64 + 0.001 * g64 * g64) + 1e-30)
+    # the update from the kernel's own moments: FP32 rounding of the bias
+    # corrections, products, square root, divide (about 8 roundings) and of the
+    # new parameter
+    mh = gm.astype(np.float64) / (1 - 0.9 ** step)
+    vh = gv.astype(np.float64) / (1 - 0.999 ** step)
+    upd = 3e-3 * mh / (np.sqrt(vh) + 1e-8)
+    assert np.all(np.abs((p64 - gp) - upd) <= 12 * u * np.abs(upd) + 2 * u * np.abs(p64))
+    # where the moments do not cancel, the parameters follow the oracle's update
+    ok = np.abs(wm) > 0.5 * (0.9 * np.abs(m64) + 0.1 * np.abs(g64))
+    assert ok.sum() > cm.NPARAM // 2
+    assert np.all(np.abs(gp - wp)[ok] <= 24 * u * np.abs(p64 - wp)[ok] + 2 * u * np.abs(p64)[ok])
+
+
 def test_training_lowers_the_loss(engine):
     rng = np.random.default_rng(11)
     n = 20000
