@@ -882,6 +882,7 @@ struct TunedRun {
     int nitems = 0;
     int fetch = 0;                        // work items per queue atomic
     int global_hist = 0;                  // 1: histogram in global memory (too large for LDS)
+    int drain_pairs = 0;                  // survivors per lane per drain of the tuned kernel
     unsigned long long* ticks = nullptr;  // per cell pair wall-clock ticks (device), when timed
     int* cu = nullptr;                    // compute unit of the last item of each cell pair (device)
 };
@@ -906,7 +907,7 @@ static void ensure_sample_store()
 struct TunedShape {
     const void* kfn = nullptr;
     bool gh = false;
-    int threads = 0, nwaves = 0, per_cu = 0;
+    int threads = 0, nwaves = 0, per_cu = 0, dp = 1;
     size_t lds = 0;
 };
 
@@ -924,20 +925,35 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
     const bool gh = !batch && (size_t)nslots * per_slot + lds_fixed_for(4) > E.lds_max;
     if (lds_fixed_for(4) > E.lds_max || (batch && (size_t)nslots * per_slot * 4 + lds_fixed_for(4) > E.lds_max))
         return kNoTunedFit;
-    const void* kfn = nullptr;
+    // the kernel of this mode and rounding order with 1 or 2 survivors per lane per drain
+    auto kernel_for = [&](int dp) -> const void* {
+        const void* k = nullptr;
+#define CF_K2(M, F, B, G)                                                                                      \
+    k = dp == 2 ? (const void*)&k_pairs_tuned<M, F, B, G, 2> : (const void*)&k_pairs_tuned<M, F, B, G, 1>;
 #define CF_K(M, F, B)                                                                                          \
-    if (P.mode == M && P.r2form == F && batch == B)                                                            \
-        kfn = gh ? (const void*)&k_pairs_tuned<M, F, B, true> : (const void*)&k_pairs_tuned<M, F, B, false>;
-    CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
-    CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
-    CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
-    CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
-    CF_K(MODE_RPPI_MOCKS, R2_SUM_XY_Z, 0) CF_K(MODE_SMU_MOCKS, R2_SUM_XY_Z, 0)
+    if (P.mode == M && P.r2form == F && batch == B) {                                                          \
+        if (gh) {                                                                                              \
+            CF_K2(M, F, B, true)                                                                               \
+        } else {                                                                                               \
+            CF_K2(M, F, B, false)                                                                              \
+        }                                                                                                      \
+    }
+        CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
+        CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
+        CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
+        CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
+        CF_K(MODE_RPPI_MOCKS, R2_SUM_XY_Z, 0) CF_K(MODE_SMU_MOCKS, R2_SUM_XY_Z, 0)
+        if (batch && P.mode == MODE_DD && P.r2form == R2_SUM_XY_Z) {
+            CF_K2(MODE_DD, R2_SUM_XY_Z, true, false)
+        }
+        if (batch && P.mode == MODE_DD && P.r2form == R2_FMA_DZ_DY_DX) {
+            CF_K2(MODE_DD, R2_FMA_DZ_DY_DX, true, false)
+        }
 #undef CF_K
-    if (batch && P.mode == MODE_DD && P.r2form == R2_SUM_XY_Z) kfn = (const void*)&k_pairs_tuned<MODE_DD, R2_SUM_XY_Z, true, false>;
-    if (batch && P.mode == MODE_DD && P.r2form == R2_FMA_DZ_DY_DX)
-        kfn = (const void*)&k_pairs_tuned<MODE_DD, R2_FMA_DZ_DY_DX, true, false>;
-    if (!kfn) {
+#undef CF_K2
+        return k;
+    };
+    if (!kernel_for(1)) {
         fprintf(stderr, "Error: no pair kernel for mode %d / rounding order %d%s\n", P.mode, P.r2form,
                 batch ? " (batched)" : "");
         return EXIT_FAILURE;
@@ -955,7 +971,7 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
         int threads, nwaves, R, prim, per_cu;
         size_t lds;
     };
-    auto shape_for = [&](int thr) {
+    auto shape_for = [&](const void* kfn, int thr) {
         Shape sh;
         sh.threads = thr;
         sh.nwaves = thr / 64;
@@ -996,12 +1012,31 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
     };
     // workgroups of 8 waves (a histogram shared by more waves; measured as fast
     // or faster), or of 4 where 8 would keep fewer waves resident or exceed LDS
-    Shape sh = shape_for(256);
-    {
-        const Shape s8 = shape_for(512);
+    auto best_shape = [&](const void* kfn) {
+        Shape sh = shape_for(kfn, 256);
+        const Shape s8 = shape_for(kfn, 512);
         if (s8.lds <= E.lds_max && s8.per_cu * s8.nwaves >= sh.per_cu * sh.nwaves) sh = s8;
         if (const char* wg = getenv("CORRFUNC_AMD_THREADS"))
-            sh = atoi(wg) == 512 && s8.lds <= E.lds_max ? s8 : shape_for(256);
+            sh = atoi(wg) == 512 && s8.lds <= E.lds_max ? s8 : shape_for(kfn, 256);
+        return sh;
+    };
+    // two survivors per lane per drain where that kernel keeps as many waves
+    // resident and as many histogram copies (launches whose LDS already limits
+    // the waves per SIMD to five or fewer; measured: DDsmu 2.5 % and DDrppi
+    // 4.5 % faster there, DD and xi 5-6 % slower where six waves fit)
+    int dp = 1;
+    const void* kfn = kernel_for(1);
+    Shape sh = best_shape(kfn);
+    {
+        const void* k2 = kernel_for(2);
+        const Shape s2 = best_shape(k2);
+        bool two = s2.per_cu * s2.nwaves >= sh.per_cu * sh.nwaves && s2.R >= sh.R;
+        if (const char* d = getenv("CORRFUNC_AMD_DRAIN_PAIRS")) two = atoi(d) == 2;
+        if (two) {
+            dp = 2;
+            kfn = k2;
+            sh = s2;
+        }
     }
     P.hist_copies = sh.R;
     P.prim_cache = sh.prim;
@@ -1014,6 +1049,7 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
         CF_CHECK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
     out.kfn = kfn;
     out.gh = gh;
+    out.dp = dp;
     out.threads = sh.threads;
     out.nwaves = sh.nwaves;
     out.per_cu = sh.per_cu;
@@ -1148,6 +1184,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     T.nitems = nitems;
     T.fetch = P.fetch;
     T.global_hist = gh ? 1 : 0;
+    T.drain_pairs = ts.dp;
     T.ticks = ticks;
     T.cu = cu;
     return EXIT_SUCCESS;
@@ -1570,6 +1607,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     st.hist_copies = tuned && !T.global_hist ? P.hist_copies : 0;
     st.queue_fetch = tuned ? T.fetch : 0;
     st.global_hist = T.global_hist;
+    st.drain_pairs = tuned ? T.drain_pairs : 0;
     st.autotuned = E.last_autotuned;
     st.sched = tuned ? (E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC
                                                                                   : E.sched_kind)
@@ -1840,6 +1878,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     st.list_ms = lms;
     st.hist_copies = P.hist_copies;
     st.queue_fetch = T.fetch;
+    st.drain_pairs = T.drain_pairs;
     st.autotuned = E.last_autotuned;
     st.sched = E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC : E.sched_kind;
     E.last = st;
@@ -1868,7 +1907,7 @@ struct BatchGraph {
     BgCat* cats = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int hist_copies = 0, queue_fetch = 0, sched = CF_AMD_SCHED_NONE;
+    int hist_copies = 0, queue_fetch = 0, drain_pairs = 0, sched = CF_AMD_SCHED_NONE;
     void* alloc(size_t bytes)
     {
         void* p = nullptr;
@@ -2041,6 +2080,7 @@ static int bg_create(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     if (P.fetch <= 0) P.fetch = fetch0 > 0 ? fetch0 : 1;
     G.hist_copies = P.hist_copies;
     G.queue_fetch = P.fetch;
+    G.drain_pairs = ts.dp;
     const long blocks = (long)E.ncu * ts.per_cu;
 
     // ---- buffers
@@ -2302,6 +2342,7 @@ static int bg_launch(BatchGraph& G, std::vector<CallOut>& outs)
     st.engine = CF_AMD_ENGINE_TUNED;
     st.hist_copies = G.hist_copies;
     st.queue_fetch = G.queue_fetch;
+    st.drain_pairs = G.drain_pairs;
     st.sched = G.sched;
     if (ncat > 0) {
         st.nmesh[0] = cats[ncat - 1].g.nx; st.nmesh[1] = cats[ncat - 1].g.ny; st.nmesh[2] = cats[ncat - 1].g.nz;

@@ -712,11 +712,15 @@ template __global__ void k_pairs_plain<true>(PairParams, DevLattice, DevLattice,
 constexpr int kQueues = 8;          // work sub-queues (one counter each)
 constexpr int kCounterStride = 32;  // ints between sub-queue counters (128 B)
 constexpr int kBlockJ = 4;          // secondaries per scalar load of the sparse path
-// Survivor buffer per wave: entries (j << 6) | lane at [head, tail). A scan runs
-// while fewer than 64 entries wait (head < kCompactAt), a block adds <= 64
-// kBlockJ, and lanes without a survivor write up to 127 slots past the tail
-// (see enqueue); drains take 64 from the head; once head >= kCompactAt the
-// remaining < 64 entries move to the front.
+// Survivor buffer per wave: entries (j << 6) | lane at [head, tail). Drains take
+// 64 * DP entries from the head (DP survivors per lane: 1, or 2 with both pairs'
+// records loaded before either is decided); a scan runs while fewer than that
+// wait, a block adds <= 64 kBlockJ, and lanes without a survivor write up to 127
+// slots past the tail (see enqueue); once head >= kCompactAt the remaining
+// entries move to the front. At a block's start head <= 64 (DP = 1; head
+// advances by 64 and is reset at 128) or head = 0 (DP = 2: every drain round
+// ends with the move) and tail - head < 64 DP, so tail <= 127 and the
+// furthest write is 127 + 64 (kBlockJ - 1) + 127 < kQueueAlloc.
 constexpr int kCompactAt = 128;
 constexpr int kQueueAlloc = kCompactAt + 64 + 64 * (kBlockJ - 1) + 127;
 // Queue entries hold secondary indices relative to the start of the window
@@ -962,6 +966,44 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
     }
 }
 
+// Decides up to 128 queued survivors, two per lane: both pairs' records are
+// loaded before either is decided, so two gathers are in flight per lane (the
+// DP = 2 kernels, for launches whose LDS already limits the waves per SIMD to
+// five or fewer: the extra registers then cost no residency).
+template <int MODE, int R2F, bool GH>
+__device__ __forceinline__ void drain_queue2(const PairParams& P, const CellPairRec& cp, const CellPairRec* cpp,
+                                             const DevLattice* l1p, const double4* d4_1, const double4* d4_2,
+                                             const double3& wrap, int i0, int istride, const unsigned* qe,
+                                             const double4* prim, unsigned head, int ndrain, int jbase,
+                                             KernelLds& S)
+{
+    const int lane = lane_id();
+    const int oka = lane < ndrain, okb = lane + 64 < ndrain;
+    // entries past the tail read as 0: j = jbase, primary i0 (valid indices)
+    const unsigned ea0 = qe[head + lane], eb0 = qe[head + 64 + lane];
+    const unsigned ea = oka ? ea0 : 0u, eb = okb ? eb0 : 0u;
+    const int ja = jbase + (int)(ea >> 6), la = (int)(ea & 63);
+    const int jb = jbase + (int)(eb >> 6), lb = (int)(eb & 63);
+    const int ia = i0 + la * istride, ib = i0 + lb * istride;
+    double4 pia, pib;
+    if (P.prim_cache) {
+        pia = prim[la];
+        pib = prim[lb];
+    } else {
+        pia = d4_1[ia];
+        pib = d4_1[ib];
+        pia.x += wrap.x;
+        pia.y += wrap.y;
+        pia.z += wrap.z;
+        pib.x += wrap.x;
+        pib.y += wrap.y;
+        pib.z += wrap.z;
+    }
+    const double4 pja = d4_2[ja], pjb = d4_2[jb];
+    exact_pair<MODE, R2F, GH>(P, cpp, l1p, ia, oka, pia.x, pia.y, pia.z, pia.w, pja.x, pja.y, pja.z, pja.w, S);
+    exact_pair<MODE, R2F, GH>(P, cpp, l1p, ib, okb, pib.x, pib.y, pib.z, pib.w, pjb.x, pjb.y, pjb.z, pjb.w, S);
+}
+
 // Integer pair filter on quantized cell-local positions (secondary s, primary
 // qxy / qz). 3-D: (dqx^2 + dqy^2 + dqz^2 + qlo) mod 2^32 < qrange; projected:
 // the same on (dqx, dqy) and |dqz| < zq, as (dqz + zq - 1) unsigned < 2 zq - 1
@@ -1052,17 +1094,17 @@ __device__ __forceinline__ void qblock(const QFilter& f, const QBlock& b, int jb
 // jb. On entry and exit `a` holds the block at jb: the loop alternates between
 // two register blocks so that each block is loaded one block ahead (q is padded
 // by kQPad entries, so the look-ahead stays inside the array).
-template <int MODE, bool SAME>
+template <int MODE, bool SAME, int DRAIN>
 __device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jhi, QBlock& a, unsigned* qe,
                                      unsigned head, unsigned& tail, int jbase)
 {
     int nfull = (jhi - jb) / kBlockJ;
     for (;;) {
-        if (nfull <= 0 || tail - head >= 64) return jb;
+        if (nfull <= 0 || tail - head >= DRAIN) return jb;
         const QBlock b = qload(qs, jb + kBlockJ);
         qblock<MODE, SAME, false>(f, a, jb, kBlockJ, qe, tail, jbase);
         jb += kBlockJ;
-        if (--nfull <= 0 || tail - head >= 64) {
+        if (--nfull <= 0 || tail - head >= DRAIN) {
             a = b;
             return jb;
         }
@@ -1074,7 +1116,7 @@ __device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jh
 }
 
 // One work item of the tuned kernel (see the comment above kQueues).
-template <int MODE, int R2F, bool GH>
+template <int MODE, int R2F, bool GH, int DP>
 __device__ __forceinline__ void process_item(const PairParams& P, const DevLattice* l1p, const DevLattice* l2p,
                                              const CellPairRec& cp, const CellPairRec* cpp, const WorkItem& it,
                                              KernelLds& S, unsigned* qe, double4* prim)
@@ -1183,16 +1225,17 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     }
     KUint* qs = (KUint*)(const void*)L2.q;
     const int max_window = Pi.max_window;
+    constexpr int kDrain = 64 * DP;
     for (int wlo = jlo; wlo < jhi; wlo += max_window) {
         const int whi = jhi - wlo > max_window ? wlo + max_window : jhi;
         QBlock a = qload(qs, wlo);
         int jb = wlo;
         while (jb < whi) {
             if (same)
-                jb = qscan<MODE, true>(f, qs, jb, whi, a, qe, head, tail, wlo);
+                jb = qscan<MODE, true, kDrain>(f, qs, jb, whi, a, qe, head, tail, wlo);
             else
-                jb = qscan<MODE, false>(f, qs, jb, whi, a, qe, head, tail, wlo);
-            if (jb < whi && jb + kBlockJ > whi && tail - head < 64) {
+                jb = qscan<MODE, false, kDrain>(f, qs, jb, whi, a, qe, head, tail, wlo);
+            if (jb < whi && jb + kBlockJ > whi && tail - head < kDrain) {
                 // the last, partial block
                 if (same)
                     qblock<MODE, true, true>(f, a, jb, whi - jb, qe, tail, wlo);
@@ -1200,20 +1243,30 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
                     qblock<MODE, false, true>(f, a, jb, whi - jb, qe, tail, wlo);
                 jb = whi;
             }
-            while (tail - head >= 64) {
+            while (tail - head >= kDrain) {
                 __builtin_amdgcn_wave_barrier();
-                drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head, 64, wlo,
-                                           S);
+                if (DP == 2)
+                    drain_queue2<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                                                kDrain, wlo, S);
+                else
+                    drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                                               kDrain, wlo, S);
                 __builtin_amdgcn_wave_barrier();
-                head += 64;
+                head += kDrain;
             }
             if (head >= kCompactAt) {
-                // move the < 64 waiting entries to the front
+                // move the < kDrain waiting entries to the front
                 const unsigned n = tail - head;
-                unsigned v = 0;
-                if ((unsigned)lane < n) v = qe[head + lane];
+                unsigned v[DP];
+#pragma unroll
+                for (int p = 0; p < DP; p++) {
+                    v[p] = 0;
+                    if ((unsigned)(lane + 64 * p) < n) v[p] = qe[head + lane + 64 * p];
+                }
                 __builtin_amdgcn_wave_barrier();
-                if ((unsigned)lane < n) qe[lane] = v;
+#pragma unroll
+                for (int p = 0; p < DP; p++)
+                    if ((unsigned)(lane + 64 * p) < n) qe[lane + 64 * p] = v[p];
                 __builtin_amdgcn_wave_barrier();
                 head = 0;
                 tail = n;
@@ -1221,8 +1274,12 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         }
         if (tail != head) {
             __builtin_amdgcn_wave_barrier();
-            drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
-                                       (int)(tail - head), wlo, S);
+            if (DP == 2)
+                drain_queue2<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                                            (int)(tail - head), wlo, S);
+            else
+                drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                                           (int)(tail - head), wlo, S);
             __builtin_amdgcn_wave_barrier();
         }
         head = tail = 0;
@@ -1257,8 +1314,10 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
 // catalog it is working on, and the histogram of catalog k lives at
 // g_*[k*nslots ...]. GH = true: no LDS histogram; every counted pair is added
 // to the global histogram (histograms too large for the workgroup's LDS).
-template <int MODE, int R2F, bool BATCH, bool GH>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
+// DP = survivors per lane per drain (1: compiled for six waves per SIMD, <= 80
+// VGPRs; 2: for five, <= 96 VGPRs).
+template <int MODE, int R2F, bool BATCH, bool GH, int DP>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DP == 2 ? 5 : 6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
                                                      const WorkItem* __restrict__ items,
                                                      const int* __restrict__ nitems_p, int* __restrict__ counter,
@@ -1379,7 +1438,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
         // (branch-free in the lanes: lane 0 adds dt, the others 0; every lane
         // stores the same compute-unit id)
         const unsigned long long t0 = cp_ticks ? wall_clock64() : 0ull;
-        process_item<MODE, R2F, GH>(P, l1p, l2p, cp, cpp, it, S, qe, prim);
+        process_item<MODE, R2F, GH, DP>(P, l1p, l2p, cp, cpp, it, S, qe, prim);
         if (cp_ticks) {
             const unsigned long long dt = wall_clock64() - t0;
             atomicAdd(&cp_ticks[it.cp], lane == 0 ? dt : 0ull);
@@ -1398,11 +1457,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     flush_region(P, S, R, 0, threadIdx.x, blockDim.x, g_np, g_avg, g_w);
 }
 
-#define CF_TUNED_INST(M, F, B, G)                                                                              \
-    template __global__ void k_pairs_tuned<M, F, B, G>(PairParams, const DevLattice*, const CellPairRec*,      \
-                                                       const WorkItem*, const int*, int*, const double*,       \
-                                                       const unsigned short*, unsigned long long*, double*,    \
-                                                       double*, unsigned long long*, int*, const int*);
+#define CF_TUNED_INST1(M, F, B, G, D)                                                                          \
+    template __global__ void k_pairs_tuned<M, F, B, G, D>(PairParams, const DevLattice*, const CellPairRec*,   \
+                                                          const WorkItem*, const int*, int*, const double*,    \
+                                                          const unsigned short*, unsigned long long*, double*, \
+                                                          double*, unsigned long long*, int*, const int*);
+#define CF_TUNED_INST(M, F, B, G) CF_TUNED_INST1(M, F, B, G, 1) CF_TUNED_INST1(M, F, B, G, 2)
 #define CF_TUNED_INST2(M, F) CF_TUNED_INST(M, F, false, false) CF_TUNED_INST(M, F, false, true)
 CF_TUNED_INST2(MODE_DD, R2_SUM_XY_Z)
 CF_TUNED_INST2(MODE_DD, R2_FMA_DZ_DY_DX)
@@ -1421,6 +1481,7 @@ CF_TUNED_INST(MODE_DD, R2_SUM_XY_Z, true, false)
 CF_TUNED_INST(MODE_DD, R2_FMA_DZ_DY_DX, true, false)
 #undef CF_TUNED_INST2
 #undef CF_TUNED_INST
+#undef CF_TUNED_INST1
 
 // Largest squared separation between the bounding boxes of a cell pair
 // (primary box shifted by the wrap), per the geometry of the mode, and whether

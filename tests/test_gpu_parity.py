@@ -102,11 +102,15 @@ def test_device_resident_inputs(engine, oracle, eng, dtype):
     assert_same("countpairs", got, want)
 
 
+@pytest.mark.parametrize("drain", ["auto", "1", "2"])
 @pytest.mark.parametrize("weighted", [False, True])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
-def test_batch_matches_single_calls_and_oracle(engine, oracle, weighted, dtype):
-    """cf_amd_countpairs_batch: each catalog equals countpairs() on it and the oracle."""
+def test_batch_matches_single_calls_and_oracle(engine, oracle, monkeypatch, weighted, dtype, drain):
+    """cf_amd_countpairs_batch: each catalog equals countpairs() on it and the oracle
+    (with either form of the batched kernel's survivor drain)."""
     _select("tuned")
+    if drain != "auto":
+        monkeypatch.setenv("CORRFUNC_AMD_DRAIN_PAIRS", drain)
     L = 120.0
     rng = np.random.default_rng(77)
     sizes = [int(v) for v in np.exp(rng.uniform(np.log(300), np.log(20000), 12))]
@@ -116,6 +120,8 @@ def test_batch_matches_single_calls_and_oracle(engine, oracle, weighted, dtype):
     bins = np.logspace(np.log10(0.5), np.log10(18.0), 12)
     o = dict(periodic=True, boxsize=L, need_avg_sep=True, isa="fallback")
     outs = _lib.countpairs_batch([tuple(c) for c in cats], bins, options=_opts(**o), weights=ws)
+    if drain != "auto":
+        assert _lib.last_stats()["drain_pairs"] == int(drain)
     for k, c in enumerate(cats):
         kw = dict(autocorr=1, nthreads=2, binfile=bins, X1=c[0], Y1=c[1], Z1=c[2])
         if weighted:
@@ -150,6 +156,10 @@ KNOBS = [
     {"CORRFUNC_AMD_QBLOCK_LOG2": "5"},
     {"CORRFUNC_AMD_QBLOCK_LOG2": "3", "CORRFUNC_AMD_FETCH": "4", "CORRFUNC_AMD_HEAVY": "1e9"},
     {"CORRFUNC_AMD_QBLOCK_LOG2": "12", "CORRFUNC_AMD_SPLIT": "64", "CORRFUNC_AMD_HEAVY": "1"},
+    {"CORRFUNC_AMD_DRAIN_PAIRS": "2"},
+    {"CORRFUNC_AMD_DRAIN_PAIRS": "2", "CORRFUNC_AMD_DENSE_FRAC": "2", "CORRFUNC_AMD_PRIM_CACHE": "0"},
+    {"CORRFUNC_AMD_DRAIN_PAIRS": "2", "CORRFUNC_AMD_PRIM_CACHE": "1", "CORRFUNC_AMD_FETCH": "4"},
+    {"CORRFUNC_AMD_DRAIN_PAIRS": "1", "CORRFUNC_AMD_HIST_COPIES": "1"},
 ]
 
 
@@ -172,6 +182,8 @@ def test_engine_variables_keep_counts(engine, oracle, knobs, mode, sched):
     try:
         os.environ.update(knobs)
         got = run(engine, mode, options=_opts(**o), **kw)
+        if "CORRFUNC_AMD_DRAIN_PAIRS" in knobs:
+            assert _lib.last_stats()["drain_pairs"] == int(knobs["CORRFUNC_AMD_DRAIN_PAIRS"])
     finally:
         for k, v in old.items():
             if v is None:
