@@ -505,7 +505,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="default: 2 for workloads 3 and 5 (seconds per plain step), else max(1, steps // 2)")
-    ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU-baseline runs (the median is reported)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU-baseline runs (the median is reported)")
     ap.add_argument("--n", type=int, default=0, help="override the workload's point count (smaller test runs)")
     ap.add_argument("--sched", default="learned", choices=["none", "heuristic", "learned"])
     ap.add_argument("--sched-ab", action="store_true", help="also time the pair kernel under every scheduler")
