@@ -219,15 +219,21 @@ static void on_interrupt(int signo) { g_interrupt_signo = signo; }
 
 struct InterruptScope {
     static constexpr int kSignals[3] = {SIGTERM, SIGINT, SIGHUP};
-    void (*prev[3])(int) = {SIG_ERR, SIG_ERR, SIG_ERR};
+    struct sigaction prev[3];  // the caller's dispositions, flags and masks, restored exactly
+    bool saved[3] = {false, false, false};
     bool outer;
     InterruptScope() : outer(g_interrupt_depth++ == 0)
     {
         if (!outer) return;
         g_interrupt_signo = 0;
+        struct sigaction sa;
+        memset(&sa, 0, sizeof sa);
+        sa.sa_handler = on_interrupt;
+        sigemptyset(&sa.sa_mask);
+        sa.sa_flags = SA_RESTART;
         for (int i = 0; i < 3; i++) {
-            prev[i] = signal(kSignals[i], on_interrupt);
-            if (prev[i] == SIG_ERR) fprintf(stderr, "Can not handle signal = %d\n", kSignals[i]);
+            saved[i] = sigaction(kSignals[i], &sa, &prev[i]) == 0;
+            if (!saved[i]) fprintf(stderr, "Can not handle signal = %d\n", kSignals[i]);
         }
     }
     ~InterruptScope()
@@ -235,7 +241,7 @@ struct InterruptScope {
         g_interrupt_depth--;
         if (!outer) return;
         for (int i = 0; i < 3; i++)
-            if (prev[i] != SIG_ERR && signal(kSignals[i], prev[i]) == SIG_ERR)
+            if (saved[i] && sigaction(kSignals[i], &prev[i], nullptr) != 0)
                 fprintf(stderr, "Could not reset signal handler for signal = %d\n", kSignals[i]);
     }
 };
