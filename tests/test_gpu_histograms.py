@@ -22,15 +22,21 @@ from helpers import assert_same, clustered, mode_kwargs, rel_err, run, uniform, 
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = {"tuned": (_lib.ENGINE_TUNED, "0"), "dense": (_lib.ENGINE_TUNED, "1"),
-           "sparse": (_lib.ENGINE_TUNED, "2"), "plain": (_lib.ENGINE_PLAIN, "0")}
+# (pair kernel, dense mode, survivors per lane per drain or None for the engine's choice)
+ENGINES = {"tuned": (_lib.ENGINE_TUNED, "0", None), "dense": (_lib.ENGINE_TUNED, "1", None),
+           "sparse": (_lib.ENGINE_TUNED, "2", None), "sparse_two": (_lib.ENGINE_TUNED, "2", "2"),
+           "sparse_one": (_lib.ENGINE_TUNED, "2", "1"), "plain": (_lib.ENGINE_PLAIN, "0", None)}
 
 
 @pytest.fixture
 def select(monkeypatch):
     def sel(eng):
-        kind, dense = ENGINES[eng]
+        kind, dense, drain = ENGINES[eng]
         monkeypatch.setenv("CORRFUNC_AMD_DENSE", dense)
+        if drain:
+            monkeypatch.setenv("CORRFUNC_AMD_DRAIN_PAIRS", drain)
+        else:
+            monkeypatch.delenv("CORRFUNC_AMD_DRAIN_PAIRS", raising=False)
         _lib.set_engine(kind)
     yield sel
     _lib.set_engine(_lib.ENGINE_TUNED)
@@ -86,7 +92,7 @@ def test_large_histograms(engine, oracle, select, eng, case):
         assert st["hist_copies"] == (0 if where == "global" else 1)
 
 
-@pytest.mark.parametrize("eng", ["tuned", "dense", "sparse"])
+@pytest.mark.parametrize("eng", ["tuned", "dense", "sparse", "sparse_two"])
 @pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_s_mu"])
 @pytest.mark.parametrize("wkind", ["wide", "signed", "zero", "outlier"])
 def test_fixed_point_weight_sums(engine, oracle, select, eng, mode, wkind):
@@ -144,8 +150,12 @@ def test_separations_far_below_their_bin_edge(engine, oracle, select, mode):
 
 @pytest.mark.parametrize("knobs", [{"CORRFUNC_AMD_MAX_WINDOW": "8"},
                                    {"CORRFUNC_AMD_MAX_WINDOW": "100", "CORRFUNC_AMD_DENSE": "2"},
-                                   {"CORRFUNC_AMD_MAX_WINDOW": "4096", "CORRFUNC_AMD_DENSE": "2"}],
-                         ids=["window_8", "window_100_sparse", "window_4096_sparse"])
+                                   {"CORRFUNC_AMD_MAX_WINDOW": "4096", "CORRFUNC_AMD_DENSE": "2"},
+                                   {"CORRFUNC_AMD_MAX_WINDOW": "100", "CORRFUNC_AMD_DENSE": "2",
+                                    "CORRFUNC_AMD_DRAIN_PAIRS": "2"},
+                                   {"CORRFUNC_AMD_MAX_WINDOW": "8", "CORRFUNC_AMD_DRAIN_PAIRS": "1"}],
+                         ids=["window_8", "window_100_sparse", "window_4096_sparse", "window_100_two_per_lane",
+                              "window_8_one_per_lane"])
 @pytest.mark.parametrize("mode", ["countpairs", "countpairs_rp_pi", "countpairs_wp", "countpairs_s_mu"])
 def test_short_windows(engine, oracle, select, monkeypatch, knobs, mode):
     """Secondary windows scanned in short sub-windows (each with its own queue
