@@ -677,6 +677,7 @@ def main():
                    "refine": list(st_t[-1][0]["refine"]), "autotuned": st_t[-1][0]["autotuned"],
                    "hist_copies": st_t[-1][0]["hist_copies"], "queue_fetch": st_t[-1][0]["queue_fetch"],
                    "drain_pairs": st_t[-1][0]["drain_pairs"],
+                   "workgroup_threads": st_t[-1][0]["workgroup_threads"],
                    "global_hist": st_t[-1][0]["global_hist"]},
         "plain_value": plain,
         "tuned_over_plain": (value / plain) if plain else None,

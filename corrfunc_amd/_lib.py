@@ -17,7 +17,8 @@ class cf_amd_stats(C.Structure):
                 ("num_work_items", C.c_int64), ("tested_pairs", C.c_double), ("kernel_ms", C.c_double),
                 ("grid_ms", C.c_double), ("total_ms", C.c_double), ("engine", C.c_int), ("sched", C.c_int),
                 ("list_ms", C.c_double), ("hist_copies", C.c_int), ("autotuned", C.c_int),
-                ("queue_fetch", C.c_int), ("global_hist", C.c_int), ("drain_pairs", C.c_int)]
+                ("queue_fetch", C.c_int), ("global_hist", C.c_int), ("drain_pairs", C.c_int),
+                ("workgroup_threads", C.c_int)]
 
 
 ENGINE_TUNED = 0
@@ -102,7 +103,8 @@ def last_stats():
                 num_work_items=s.num_work_items, tested_pairs=s.tested_pairs, kernel_ms=s.kernel_ms,
                 grid_ms=s.grid_ms, total_ms=s.total_ms, engine=s.engine, sched=s.sched, list_ms=s.list_ms,
                 hist_copies=s.hist_copies, autotuned=s.autotuned, queue_fetch=s.queue_fetch,
-                global_hist=s.global_hist, drain_pairs=s.drain_pairs)
+                global_hist=s.global_hist, drain_pairs=s.drain_pairs,
+                workgroup_threads=s.workgroup_threads)
 
 
 def set_autotune(on=True):

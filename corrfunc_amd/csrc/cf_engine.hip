@@ -889,6 +889,7 @@ struct TunedRun {
     int fetch = 0;                        // work items per queue atomic
     int global_hist = 0;                  // 1: histogram in global memory (too large for LDS)
     int drain_pairs = 0;                  // survivors per lane per drain of the tuned kernel
+    int threads = 0;                      // workgroup size of the tuned kernel
     unsigned long long* ticks = nullptr;  // per cell pair wall-clock ticks (device), when timed
     int* cu = nullptr;                    // compute unit of the last item of each cell pair (device)
 };
@@ -1010,7 +1011,7 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
         if (E.force_R > 0 && E.force_R < R) R = E.force_R;
         sh.R = R;
         const size_t lds_base = lds_for(R);
-        sh.prim = resident(lds_base + lds_prim) >= resident(lds_base) ? 1 : 0;
+        sh.prim = lds_base + lds_prim <= E.lds_max && resident(lds_base + lds_prim) >= resident(lds_base) ? 1 : 0;
         if (const char* pc = getenv("CORRFUNC_AMD_PRIM_CACHE")) sh.prim = atoi(pc) != 0;
         sh.lds = lds_base + (sh.prim ? lds_prim : 0);
         sh.per_cu = resident(sh.lds);
@@ -1022,8 +1023,12 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
         Shape sh = shape_for(kfn, 256);
         const Shape s8 = shape_for(kfn, 512);
         if (s8.lds <= E.lds_max && s8.per_cu * s8.nwaves >= sh.per_cu * sh.nwaves) sh = s8;
-        if (const char* wg = getenv("CORRFUNC_AMD_THREADS"))
-            sh = atoi(wg) == 512 && s8.lds <= E.lds_max ? s8 : shape_for(kfn, 256);
+        const Shape s16 = shape_for(kfn, 1024);
+        if (s16.lds <= E.lds_max && s16.per_cu * s16.nwaves >= sh.per_cu * sh.nwaves && s16.R > sh.R) sh = s16;
+        if (const char* wg = getenv("CORRFUNC_AMD_THREADS")) {
+            const int t = atoi(wg);
+            sh = t == 1024 && s16.lds <= E.lds_max ? s16 : (t == 512 && s8.lds <= E.lds_max ? s8 : shape_for(kfn, 256));
+        }
         return sh;
     };
     // two survivors per lane per drain where that kernel keeps as many waves
@@ -1191,6 +1196,7 @@ static int launch_tuned(int ncp, const CellPairRec* cps, const DevLattice* lats,
     T.fetch = P.fetch;
     T.global_hist = gh ? 1 : 0;
     T.drain_pairs = ts.dp;
+    T.threads = ts.threads;
     T.ticks = ticks;
     T.cu = cu;
     return EXIT_SUCCESS;
@@ -1614,6 +1620,7 @@ static int run_core(const CallSpec& c, struct config_options* o, struct extra_op
     st.queue_fetch = tuned ? T.fetch : 0;
     st.global_hist = T.global_hist;
     st.drain_pairs = tuned ? T.drain_pairs : 0;
+    st.workgroup_threads = tuned ? T.threads : 0;
     st.autotuned = E.last_autotuned;
     st.sched = tuned ? (E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC
                                                                                   : E.sched_kind)
@@ -1885,6 +1892,7 @@ static int run_batch(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     st.hist_copies = P.hist_copies;
     st.queue_fetch = T.fetch;
     st.drain_pairs = T.drain_pairs;
+    st.workgroup_threads = T.threads;
     st.autotuned = E.last_autotuned;
     st.sched = E.sched_kind == CF_AMD_SCHED_LEARNED && !E.model.trained ? CF_AMD_SCHED_HEURISTIC : E.sched_kind;
     E.last = st;
@@ -1913,7 +1921,7 @@ struct BatchGraph {
     BgCat* cats = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int hist_copies = 0, queue_fetch = 0, drain_pairs = 0, sched = CF_AMD_SCHED_NONE;
+    int hist_copies = 0, queue_fetch = 0, drain_pairs = 0, threads = 0, sched = CF_AMD_SCHED_NONE;
     void* alloc(size_t bytes)
     {
         void* p = nullptr;
@@ -2087,6 +2095,7 @@ static int bg_create(int ncat, const int64_t* ND, void** X, void** Y, void** Z, 
     G.hist_copies = P.hist_copies;
     G.queue_fetch = P.fetch;
     G.drain_pairs = ts.dp;
+    G.threads = ts.threads;
     const long blocks = (long)E.ncu * ts.per_cu;
 
     // ---- buffers
@@ -2349,6 +2358,7 @@ static int bg_launch(BatchGraph& G, std::vector<CallOut>& outs)
     st.hist_copies = G.hist_copies;
     st.queue_fetch = G.queue_fetch;
     st.drain_pairs = G.drain_pairs;
+    st.workgroup_threads = G.threads;
     st.sched = G.sched;
     if (ncat > 0) {
         st.nmesh[0] = cats[ncat - 1].g.nx; st.nmesh[1] = cats[ncat - 1].g.ny; st.nmesh[2] = cats[ncat - 1].g.nz;

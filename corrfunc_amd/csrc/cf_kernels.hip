@@ -1317,7 +1317,7 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
 // DP = survivors per lane per drain (1: compiled for six waves per SIMD, <= 80
 // VGPRs; 2: for five, <= 96 VGPRs).
 template <int MODE, int R2F, bool BATCH, bool GH, int DP>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DP == 2 ? 5 : 6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(DP == 2 ? 5 : 6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
                                                      const WorkItem* __restrict__ items,
                                                      const int* __restrict__ nitems_p, int* __restrict__ counter,

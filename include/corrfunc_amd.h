@@ -46,6 +46,7 @@ typedef struct {
     int queue_fetch;         /* work items per queue atomic of the TUNED kernel (0 for PLAIN) */
     int global_hist;         /* 1: the histogram was too large for LDS and was kept in global memory */
     int drain_pairs;         /* TUNED kernel: survivors each lane decides per drain (1 or 2; 0 for PLAIN) */
+    int workgroup_threads;   /* TUNED kernel: threads per workgroup (256, 512 or 1024; 0 for PLAIN) */
 } cf_amd_stats;
 
 /* Autotuner. When on, a call whose options carry the default (non-custom) bin

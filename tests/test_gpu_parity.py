@@ -160,6 +160,10 @@ KNOBS = [
     {"CORRFUNC_AMD_DRAIN_PAIRS": "2", "CORRFUNC_AMD_DENSE_FRAC": "2", "CORRFUNC_AMD_PRIM_CACHE": "0"},
     {"CORRFUNC_AMD_DRAIN_PAIRS": "2", "CORRFUNC_AMD_PRIM_CACHE": "1", "CORRFUNC_AMD_FETCH": "4"},
     {"CORRFUNC_AMD_DRAIN_PAIRS": "1", "CORRFUNC_AMD_HIST_COPIES": "1"},
+    {"CORRFUNC_AMD_THREADS": "1024"},
+    {"CORRFUNC_AMD_THREADS": "1024", "CORRFUNC_AMD_DRAIN_PAIRS": "1", "CORRFUNC_AMD_HIST_COPIES": "2"},
+    {"CORRFUNC_AMD_THREADS": "1024", "CORRFUNC_AMD_DRAIN_PAIRS": "2", "CORRFUNC_AMD_DENSE_FRAC": "2"},
+    {"CORRFUNC_AMD_THREADS": "256", "CORRFUNC_AMD_DRAIN_PAIRS": "2"},
 ]
 
 
@@ -168,7 +172,8 @@ KNOBS = [
 @pytest.mark.parametrize("sched", ["none", "heuristic"])
 def test_engine_variables_keep_counts(engine, oracle, knobs, mode, sched):
     """Fetch batches, sub-queue stealing, dense/sparse choice, the primaries cache,
-    histogram replication, heavy-first order and window splitting never change a count."""
+    histogram replication, workgroup size, heavy-first order and window splitting
+    never change a count."""
     from corrfunc_amd import scheduler
     _select("tuned")
     L = 160.0
@@ -184,6 +189,8 @@ def test_engine_variables_keep_counts(engine, oracle, knobs, mode, sched):
         got = run(engine, mode, options=_opts(**o), **kw)
         if "CORRFUNC_AMD_DRAIN_PAIRS" in knobs:
             assert _lib.last_stats()["drain_pairs"] == int(knobs["CORRFUNC_AMD_DRAIN_PAIRS"])
+        if "CORRFUNC_AMD_THREADS" in knobs:
+            assert _lib.last_stats()["workgroup_threads"] == int(knobs["CORRFUNC_AMD_THREADS"])
     finally:
         for k, v in old.items():
             if v is None:
