@@ -1000,6 +1000,7 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
         int R = 1;
         for (int r = 2; r <= 32 && !gh; r *= 2) {
             if ((size_t)regions * nslots * per_slot * r > E.lds_max - 24 * 1024) break;
+            if (lds_for(r) > E.lds_max) break;  // (the workgroup's queues included)
             if (resident(lds_for(r)) * sh.nwaves < target) break;
             R = r;
         }
