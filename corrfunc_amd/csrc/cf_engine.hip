@@ -918,6 +918,31 @@ struct TunedShape {
     size_t lds = 0;
 };
 
+// The tuned kernel of mode M, rounding order F and batch form B for a launch:
+// the global-histogram form (gh; runtime flags, not batched) or the
+// LDS-histogram form of flag mask fl, with dp survivors per lane per drain.
+template <int M, int F, bool B>
+static const void* tuned_kernel(bool gh, int dp, int fl)
+{
+    if (gh) {
+        if constexpr (B) {
+            return nullptr;
+        } else {
+            return dp == 2 ? (const void*)&k_pairs_tuned<M, F, false, true, 2, kFlRuntime>
+                           : (const void*)&k_pairs_tuned<M, F, false, true, 1, kFlRuntime>;
+        }
+    }
+#define CF_FL(X)                                                                                               \
+    case X:                                                                                                    \
+        return dp == 2 ? (const void*)&k_pairs_tuned<M, F, B, false, 2, X>                                     \
+                       : (const void*)&k_pairs_tuned<M, F, B, false, 1, X>;
+    switch (fl) {
+        CF_FL(0) CF_FL(1) CF_FL(2) CF_FL(3) CF_FL(4) CF_FL(5) CF_FL(6) CF_FL(7)
+    }
+#undef CF_FL
+    return nullptr;
+}
+
 static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShape& out)
 {
     Engine& E = eng();
@@ -933,31 +958,25 @@ static int tuned_shape(int batch, PairParams& P, int nslots, int nbin, TunedShap
     if (lds_fixed_for(4) > E.lds_max || (batch && (size_t)nslots * per_slot * 4 + lds_fixed_for(4) > E.lds_max))
         return kNoTunedFit;
     // the kernel of this mode and rounding order with 1 or 2 survivors per lane per drain
+    // the launch's flags as the LDS-histogram kernels take them (compile-time
+    // sums and bin lookup; the AVX512F index rule follows the rounding order)
+    const int fl = (P.need_avg ? kFlAvg : 0) | (P.need_w ? kFlW : 0) | (P.sorted_edges && P.lut_single ? kFlLut1 : 0);
+    const bool findex = (P.mode == MODE_RPPI && P.r2form == RP_FMA) || (P.mode == MODE_SMU && P.r2form == R2_FMA_DX_DY_DZ);
+    if ((P.float_index != 0) != findex) {
+        fprintf(stderr, "Error: index rule %d does not match rounding order %d\n", P.float_index, P.r2form);
+        return EXIT_FAILURE;
+    }
     auto kernel_for = [&](int dp) -> const void* {
         const void* k = nullptr;
-#define CF_K2(M, F, B, G)                                                                                      \
-    k = dp == 2 ? (const void*)&k_pairs_tuned<M, F, B, G, 2> : (const void*)&k_pairs_tuned<M, F, B, G, 1>;
 #define CF_K(M, F, B)                                                                                          \
-    if (P.mode == M && P.r2form == F && batch == B) {                                                          \
-        if (gh) {                                                                                              \
-            CF_K2(M, F, B, true)                                                                               \
-        } else {                                                                                               \
-            CF_K2(M, F, B, false)                                                                              \
-        }                                                                                                      \
-    }
+    if (P.mode == M && P.r2form == F && batch == B) k = tuned_kernel<M, F, B>(gh, dp, fl);
         CF_K(MODE_DD, R2_SUM_XY_Z, 0) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 0)
         CF_K(MODE_XI, R2_SUM_XY_Z, 0) CF_K(MODE_XI, R2_X_SUM_YZ, 0) CF_K(MODE_XI, R2_FMA_DZ_DY_DX, 0)
         CF_K(MODE_RPPI, RP_SUM, 0) CF_K(MODE_RPPI, RP_FMA, 0) CF_K(MODE_WP, RP_SUM, 0) CF_K(MODE_WP, RP_FMA, 0)
         CF_K(MODE_SMU, R2_SUM_XY_Z, 0) CF_K(MODE_SMU, R2_FMA_DX_DY_DZ, 0)
         CF_K(MODE_RPPI_MOCKS, R2_SUM_XY_Z, 0) CF_K(MODE_SMU_MOCKS, R2_SUM_XY_Z, 0)
-        if (batch && P.mode == MODE_DD && P.r2form == R2_SUM_XY_Z) {
-            CF_K2(MODE_DD, R2_SUM_XY_Z, true, false)
-        }
-        if (batch && P.mode == MODE_DD && P.r2form == R2_FMA_DZ_DY_DX) {
-            CF_K2(MODE_DD, R2_FMA_DZ_DY_DX, true, false)
-        }
+        CF_K(MODE_DD, R2_SUM_XY_Z, 1) CF_K(MODE_DD, R2_FMA_DZ_DY_DX, 1)
 #undef CF_K
-#undef CF_K2
         return k;
     };
     if (!kernel_for(1)) {

@@ -775,32 +775,65 @@ struct KernelLds {
     double* gw;
 };
 
+// Per-launch flags of the tuned kernel as compile-time constants: FL >= 0 is the
+// mask of the launch's separation sums (kFlAvg), weight sums (kFlW) and
+// single-step bin lookup (kFlLut1: sorted edges, at most one edge per lookup
+// bucket); FL = kFlRuntime reads them from P (the global-histogram kernels).
+// Read from P, every such test in the per-pair code is a uniform branch on a
+// scalar lane mask held across the persistent loop, and the masks crowd the
+// scalar registers into spills.
+constexpr int kFlAvg = 1, kFlW = 2, kFlLut1 = 4, kFlRuntime = -1;
+template <int FL>
+__device__ __forceinline__ bool fl_avg(const PairParams& P)
+{
+    return FL < 0 ? P.need_avg != 0 : (FL & kFlAvg) != 0;
+}
+template <int FL>
+__device__ __forceinline__ bool fl_w(const PairParams& P)
+{
+    return FL < 0 ? P.need_w != 0 : (FL & kFlW) != 0;
+}
+template <int FL>
+__device__ __forceinline__ bool fl_lut1(const PairParams& P)
+{
+    return FL < 0 ? (P.sorted_edges != 0 && P.lut_single != 0) : (FL & kFlLut1) != 0;
+}
+// The reference's AVX512F flattened (rp, pi) / (s, mu) index rule: used exactly
+// when the rounding order replayed is the AVX512F kernel's (the host sets
+// P.float_index by the same rule).
+template <int MODE, int R2F>
+__device__ __forceinline__ constexpr bool float_index_of()
+{
+    return (MODE == MODE_RPPI && R2F == RP_FMA) || (MODE == MODE_SMU && R2F == R2_FMA_DX_DY_DZ);
+}
+
 // Adds a counted pair (slot, squared separation r2, weight product) to the histogram.
-template <bool GH>
+template <bool GH, int FL>
 __device__ __forceinline__ void hist_pair(const PairParams& P, KernelLds& S, int slot, double r2, double wprod)
 {
     if (GH) {
         atomicAdd(&S.gnp[slot], 1ull);
-        if (P.need_avg) atomicAdd(&S.gavg[slot], fast_sqrt(r2));
-        if (P.need_w) atomicAdd(&S.gw[slot], wprod);
+        if (fl_avg<FL>(P)) atomicAdd(&S.gavg[slot], fast_sqrt(r2));
+        if (fl_w<FL>(P)) atomicAdd(&S.gw[slot], wprod);
         return;
     }
     const int a = slot * (S.rmask + 1) + S.lane_copy;
     atomicAdd(&S.np[a], 1ull);
-    if (P.need_avg) atomicAdd(&S.avg[a], fast_sqrt(r2));
-    if (P.need_w) atomicAdd(&S.w[a], wprod);
+    if (fl_avg<FL>(P)) atomicAdd(&S.avg[a], fast_sqrt(r2));
+    if (fl_w<FL>(P)) atomicAdd(&S.w[a], wprod);
 }
 
 // Number of edges e[0..nbin-2] <= r2 (the reference's top-down bin search,
 // countpairs_kernels.c.src:982-993, for non-decreasing edges): start from the
 // lookup table's lower bound for the 1/16-octave bucket of (float)r2 and step up.
+template <int FL>
 __device__ __forceinline__ int lut_kbin(const PairParams& P, const KernelLds& S, double r2)
 {
-    if (!P.sorted_edges) return find_kbin(S.e, P.nbin, 0, r2);
+    if (!fl_lut1<FL>(P) && !P.sorted_edges) return find_kbin(S.e, P.nbin, 0, r2);
     int idx = (int)(__float_as_uint((float)r2) >> 19) - P.lut_base;
     idx = idx < 0 ? 0 : (idx >= P.lut_n ? P.lut_n - 1 : idx);
     int k = S.lut[idx];
-    if (P.lut_single) return k + (S.e[k] <= r2 ? 1 : 0);  // e[nbin-1] = rmax^2 > r2
+    if (fl_lut1<FL>(P)) return k + (S.e[k] <= r2 ? 1 : 0);  // e[nbin-1] = rmax^2 > r2
     while (k < P.nbin - 1 && S.e[k] <= r2) k++;
     return k;
 }
@@ -838,7 +871,7 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* fresh_k(co
 // Decides one (i, j) pair in FP64 exactly as the reference does and adds it to
 // the LDS histogram. (xpos, ypos, zpos) = primary position plus the cell pair's
 // wrap; (xj, yj, zj) = secondary position. `ok` masks lanes without a pair.
-template <int MODE, int R2F, bool GH>
+template <int MODE, int R2F, bool GH, int FL>
 __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRec* cpp, const DevLattice* l1p, int i,
                                            int ok, double xpos, double ypos, double zpos, double wi, double xj,
                                            double yj, double zj, double wj, KernelLds& S)
@@ -858,9 +891,9 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
             i_terms(P, MODE, *c, l1->x[i], l1->y[i], l1->z[i], &t);
             if (t.skip || zj <= t.target || dz >= t.max_dz) return;
         }
-        const int k = lut_kbin(P, S, key);
+        const int k = lut_kbin<FL>(P, S, key);
         if (k == 0) return;
-        hist_pair<GH>(P, S, k * (P.n2 + 1) + j2, key, wi * wj);
+        hist_pair<GH, FL>(P, S, k * (P.n2 + 1) + j2, key, wi * wj);
         return;
     }
     const double sqr_dz = dz * dz;
@@ -886,12 +919,13 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
             if (t.skip || zj <= t.target || dz >= t.max_dz) return;
         }
     }
+    const bool findex = FL < 0 ? P.float_index != 0 : float_index_of<MODE, R2F>();
     int k;
-    if (P.sorted_edges) {
+    if (fl_lut1<FL>(P) || P.sorted_edges) {
         int idx = (int)(__float_as_uint((float)r2) >> 19) - P.lut_base;
         idx = idx < 0 ? 0 : (idx >= P.lut_n ? P.lut_n - 1 : idx);
         k = S.lut[idx];
-        if (P.lut_single) {
+        if (fl_lut1<FL>(P)) {
             // at most one edge above lut[idx] can lie below r2 (e[nbin-1] = rmax^2 > r2)
             k += S.e[k] <= r2 ? 1 : 0;
         } else if (k < P.nbin - 1 && S.e[k] <= r2) {
@@ -904,7 +938,7 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
     if (k == 0) return;
     int slot = k;
     if (MODE == MODE_RPPI) {
-        if (P.float_index) {
+        if (findex) {
             slot = (int)((double)k * (double)(P.n2 + 1) + adz * P.inv_d2);
         } else {
             int pb = (int)(adz * P.inv_d2);
@@ -927,7 +961,7 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
             slot = k * (P.n2 + 1) + (mbf > P.n2 ? P.n2 : mbf);
         } else {
             const double mu = sqrt(sqr_dz / r2);
-            if (P.float_index) {
+            if (findex) {
                 slot = (int)((double)k * (double)(P.n2 + 1) + mu * P.inv_d2);
             } else {
                 int mb = (int)(mu * P.inv_d2);
@@ -936,11 +970,11 @@ __device__ __forceinline__ void exact_pair(const PairParams& P, const CellPairRe
             }
         }
     }
-    hist_pair<GH>(P, S, slot, r2, wi * wj);
+    hist_pair<GH, FL>(P, S, slot, r2, wi * wj);
 }
 
 // Decides up to 64 queued survivors (entries ((j - jbase) << 6) | lane) in FP64.
-template <int MODE, int R2F, bool GH>
+template <int MODE, int R2F, bool GH, int FL>
 __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairRec& cp, const CellPairRec* cpp,
                                             const DevLattice* l1p, const double4* d4_1, const double4* d4_2,
                                             const double3& wrap, int i0, int istride, const unsigned* qe,
@@ -962,7 +996,7 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
             pi.z += wrap.z;
         }
         const double4 pj = d4_2[j];
-        exact_pair<MODE, R2F, GH>(P, cpp, l1p, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
+        exact_pair<MODE, R2F, GH, FL>(P, cpp, l1p, i, 1, pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, S);
     }
 }
 
@@ -970,7 +1004,7 @@ __device__ __forceinline__ void drain_queue(const PairParams& P, const CellPairR
 // loaded before either is decided, so two gathers are in flight per lane (the
 // DP = 2 kernels, for launches whose LDS already limits the waves per SIMD to
 // five or fewer: the extra registers then cost no residency).
-template <int MODE, int R2F, bool GH>
+template <int MODE, int R2F, bool GH, int FL>
 __device__ __forceinline__ void drain_queue2(const PairParams& P, const CellPairRec& cp, const CellPairRec* cpp,
                                              const DevLattice* l1p, const double4* d4_1, const double4* d4_2,
                                              const double3& wrap, int i0, int istride, const unsigned* qe,
@@ -1000,8 +1034,8 @@ __device__ __forceinline__ void drain_queue2(const PairParams& P, const CellPair
         pib.z += wrap.z;
     }
     const double4 pja = d4_2[ja], pjb = d4_2[jb];
-    exact_pair<MODE, R2F, GH>(P, cpp, l1p, ia, oka, pia.x, pia.y, pia.z, pia.w, pja.x, pja.y, pja.z, pja.w, S);
-    exact_pair<MODE, R2F, GH>(P, cpp, l1p, ib, okb, pib.x, pib.y, pib.z, pib.w, pjb.x, pjb.y, pjb.z, pjb.w, S);
+    exact_pair<MODE, R2F, GH, FL>(P, cpp, l1p, ia, oka, pia.x, pia.y, pia.z, pia.w, pja.x, pja.y, pja.z, pja.w, S);
+    exact_pair<MODE, R2F, GH, FL>(P, cpp, l1p, ib, okb, pib.x, pib.y, pib.z, pib.w, pjb.x, pjb.y, pjb.z, pjb.w, S);
 }
 
 // Integer pair filter on quantized cell-local positions (secondary s, primary
@@ -1116,7 +1150,7 @@ __device__ __forceinline__ int qscan(const QFilter& f, KUint* qs, int jb, int jh
 }
 
 // One work item of the tuned kernel (see the comment above kQueues).
-template <int MODE, int R2F, bool GH, int DP>
+template <int MODE, int R2F, bool GH, int DP, int FL>
 __device__ __forceinline__ void process_item(const PairParams& P, const DevLattice* l1p, const DevLattice* l2p,
                                              const CellPairRec& cp, const CellPairRec* cpp, const WorkItem& it,
                                              KernelLds& S, unsigned* qe, double4* prim)
@@ -1192,8 +1226,8 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         double4 cur = kload_d4(ds, jlo);
         for (int j = jlo; j < jhi; j++) {
             const double4 nxt = kload_d4(ds, j + 1);
-            exact_pair<MODE, R2F, GH>(P, cpp, l1p, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
-                                      P.need_w ? cur.w : 0.0, S);
+            exact_pair<MODE, R2F, GH, FL>(P, cpp, l1p, i, j > ilim, xpos, ypos, zpos, wi, cur.x, cur.y, cur.z,
+                                          fl_w<FL>(P) ? cur.w : 0.0, S);
             cur = nxt;
         }
         return;
@@ -1246,10 +1280,10 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
             while (tail - head >= kDrain) {
                 __builtin_amdgcn_wave_barrier();
                 if (DP == 2)
-                    drain_queue2<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                    drain_queue2<MODE, R2F, GH, FL>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
                                                 kDrain, wlo, S);
                 else
-                    drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                    drain_queue<MODE, R2F, GH, FL>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
                                                kDrain, wlo, S);
                 __builtin_amdgcn_wave_barrier();
                 head += kDrain;
@@ -1275,10 +1309,10 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
         if (tail != head) {
             __builtin_amdgcn_wave_barrier();
             if (DP == 2)
-                drain_queue2<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                drain_queue2<MODE, R2F, GH, FL>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
                                             (int)(tail - head), wlo, S);
             else
-                drain_queue<MODE, R2F, GH>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
+                drain_queue<MODE, R2F, GH, FL>(P, cp, cpp, l1p, d4_1, d4_2, wrap, i0, istride, qe, prim, head,
                                            (int)(tail - head), wlo, S);
             __builtin_amdgcn_wave_barrier();
         }
@@ -1315,8 +1349,9 @@ __device__ __forceinline__ void flush_region(const PairParams& P, KernelLds& S, 
 // g_*[k*nslots ...]. GH = true: no LDS histogram; every counted pair is added
 // to the global histogram (histograms too large for the workgroup's LDS).
 // DP = survivors per lane per drain (1: compiled for six waves per SIMD, <= 80
-// VGPRs; 2: for five, <= 96 VGPRs).
-template <int MODE, int R2F, bool BATCH, bool GH, int DP>
+// VGPRs; 2: for five, <= 96 VGPRs). FL = the launch's flags (fl_avg etc.):
+// a mask for the LDS-histogram kernels, kFlRuntime for the GH ones.
+template <int MODE, int R2F, bool BATCH, bool GH, int DP, int FL>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(DP == 2 ? 5 : 6))) void k_pairs_tuned(PairParams P, const DevLattice* __restrict__ lats,
                                                      const CellPairRec* __restrict__ cps,
                                                      const WorkItem* __restrict__ items,
@@ -1438,7 +1473,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(DP == 2 ? 
         // (branch-free in the lanes: lane 0 adds dt, the others 0; every lane
         // stores the same compute-unit id)
         const unsigned long long t0 = cp_ticks ? wall_clock64() : 0ull;
-        process_item<MODE, R2F, GH, DP>(P, l1p, l2p, cp, cpp, it, S, qe, prim);
+        process_item<MODE, R2F, GH, DP, FL>(P, l1p, l2p, cp, cpp, it, S, qe, prim);
         if (cp_ticks) {
             const unsigned long long dt = wall_clock64() - t0;
             atomicAdd(&cp_ticks[it.cp], lane == 0 ? dt : 0ull);
@@ -1457,13 +1492,22 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(DP == 2 ? 
     flush_region(P, S, R, 0, threadIdx.x, blockDim.x, g_np, g_avg, g_w);
 }
 
-#define CF_TUNED_INST1(M, F, B, G, D)                                                                          \
-    template __global__ void k_pairs_tuned<M, F, B, G, D>(PairParams, const DevLattice*, const CellPairRec*,   \
-                                                          const WorkItem*, const int*, int*, const double*,    \
-                                                          const unsigned short*, unsigned long long*, double*, \
-                                                          double*, unsigned long long*, int*, const int*);
-#define CF_TUNED_INST(M, F, B, G) CF_TUNED_INST1(M, F, B, G, 1) CF_TUNED_INST1(M, F, B, G, 2)
-#define CF_TUNED_INST2(M, F) CF_TUNED_INST(M, F, false, false) CF_TUNED_INST(M, F, false, true)
+#define CF_TUNED_INST0(M, F, B, G, D, FL)                                                                      \
+    template __global__ void k_pairs_tuned<M, F, B, G, D, FL>(PairParams, const DevLattice*, const CellPairRec*, \
+                                                              const WorkItem*, const int*, int*, const double*,  \
+                                                              const unsigned short*, unsigned long long*,         \
+                                                              double*, double*, unsigned long long*, int*,        \
+                                                              const int*);
+// LDS-histogram kernels: one per flag mask; global-histogram kernels: runtime flags
+#define CF_TUNED_INST_FL(M, F, B, D)                                                                           \
+    CF_TUNED_INST0(M, F, B, false, D, 0) CF_TUNED_INST0(M, F, B, false, D, 1)                                   \
+    CF_TUNED_INST0(M, F, B, false, D, 2) CF_TUNED_INST0(M, F, B, false, D, 3)                                   \
+    CF_TUNED_INST0(M, F, B, false, D, 4) CF_TUNED_INST0(M, F, B, false, D, 5)                                   \
+    CF_TUNED_INST0(M, F, B, false, D, 6) CF_TUNED_INST0(M, F, B, false, D, 7)
+#define CF_TUNED_INST(M, F, B) CF_TUNED_INST_FL(M, F, B, 1) CF_TUNED_INST_FL(M, F, B, 2)
+#define CF_TUNED_INST2(M, F)                                                                                   \
+    CF_TUNED_INST(M, F, false) CF_TUNED_INST0(M, F, false, true, 1, kFlRuntime)                                 \
+    CF_TUNED_INST0(M, F, false, true, 2, kFlRuntime)
 CF_TUNED_INST2(MODE_DD, R2_SUM_XY_Z)
 CF_TUNED_INST2(MODE_DD, R2_FMA_DZ_DY_DX)
 CF_TUNED_INST2(MODE_XI, R2_SUM_XY_Z)
@@ -1477,11 +1521,12 @@ CF_TUNED_INST2(MODE_SMU, R2_SUM_XY_Z)
 CF_TUNED_INST2(MODE_SMU, R2_FMA_DX_DY_DZ)
 CF_TUNED_INST2(MODE_RPPI_MOCKS, R2_SUM_XY_Z)
 CF_TUNED_INST2(MODE_SMU_MOCKS, R2_SUM_XY_Z)
-CF_TUNED_INST(MODE_DD, R2_SUM_XY_Z, true, false)
-CF_TUNED_INST(MODE_DD, R2_FMA_DZ_DY_DX, true, false)
+CF_TUNED_INST(MODE_DD, R2_SUM_XY_Z, true)
+CF_TUNED_INST(MODE_DD, R2_FMA_DZ_DY_DX, true)
 #undef CF_TUNED_INST2
 #undef CF_TUNED_INST
-#undef CF_TUNED_INST1
+#undef CF_TUNED_INST_FL
+#undef CF_TUNED_INST0
 
 // Largest squared separation between the bounding boxes of a cell pair
 // (primary box shifted by the wrap), per the geometry of the mode, and whether
