@@ -1164,8 +1164,9 @@ __device__ __forceinline__ void process_item(const PairParams& P, const DevLatti
     const int i0 = __builtin_amdgcn_readfirstlane(it.i0), i1 = __builtin_amdgcn_readfirstlane(it.i1);
     const int flags = __builtin_amdgcn_readfirstlane(it.flags);
     const int dense = Pi.dense_mode == 1 ? 1 : (Pi.dense_mode == 2 ? 0 : (flags & kItemDense));
-    // dense items take strided primaries (whole-cell sample per wave), which
-    // spreads the lanes' pairs over more histogram slots
+    // with P.dense_stride, dense cross-cell items take strided primaries (a
+    // whole-cell sample per wave, spreading the lanes' pairs over more
+    // histogram slots); else 64 consecutive ones
     const int stride = flags >> kStrideShift;
     const int i = i0 + lane * (stride > 0 ? stride : 1);
     const int ivalid = i < i1;

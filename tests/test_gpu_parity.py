@@ -149,6 +149,7 @@ KNOBS = [
     {"CORRFUNC_AMD_FETCH": "4"},
     {"CORRFUNC_AMD_FETCH": "3", "CORRFUNC_AMD_HEAVY": "1e9"},
     {"CORRFUNC_AMD_DENSE_FRAC": "0"},
+    {"CORRFUNC_AMD_DENSE_FRAC": "0", "CORRFUNC_AMD_STRIDE": "1"},
     {"CORRFUNC_AMD_DENSE_FRAC": "2", "CORRFUNC_AMD_PRIM_CACHE": "0"},
     {"CORRFUNC_AMD_PRIM_CACHE": "1", "CORRFUNC_AMD_HIST_COPIES": "1"},
     {"CORRFUNC_AMD_SPLIT": "64", "CORRFUNC_AMD_HEAVY": "1"},
